@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""Throughput of the HIP acoustic forward operator on one MI355X.
+
+Default workload (the headline configuration): 3D constant density, float32,
+384^3 grid at 12.5 m, space order 8, 40-point cubic damping layer
+(alpha 0.001) -> 472^3 extended grid; 6 flat velocity layers 1500..4500 m/s
+times (1 + p), p seeded Gaussian-smoothed noise (sigma 8 cells, max |p| 5%);
+tf = 1.5 s at the CFL step; one 15 Hz Ricker source at (100, 2393.75,
+2393.75) m; 128 x 128 receivers at z = 25 m; saving_stride 0.
+
+A "step" is one complete shot (every time step of the run) with the models,
+wavefield slots and shot record resident in HBM; the wavefield is zeroed at
+the start of each shot inside the timed region.  value = interior grid-point
+updates per second (Gpts/s).
+
+The roofline object is measured live for the dominant kernel (the fused
+stencil+source+boundary step kernel) with HIP events around each launch on
+the launch stream.  The CPU baseline times the reference's own cpu_openmp
+forward (built from its sources into oracle/_ref) or, where that build is
+absent, the C restatement in oracle/, on a bounded number of time steps of
+the same grid.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("Gpts/s & % HBM roofline (3D order-8 fp32); "
+          "p50/p99 shot latency under load")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ workloads
+def headline_3d(dtype=np.float32, seed=0):
+    """384^3, so8, 40-cell damping, 6 layers +-5% smooth noise."""
+    import simwave_amd as sw
+    from scipy.ndimage import gaussian_filter
+
+    n, h = 384, 12.5
+    rng = np.random.default_rng(seed)
+    noise = gaussian_filter(rng.standard_normal((n, n, n), dtype=np.float32),
+                            8.0, mode="reflect", truncate=3.0)
+    noise *= np.float32(0.05 / np.abs(noise).max())
+    layers = np.linspace(1500.0, 4500.0, 6)
+    vel = np.repeat(layers, n // 6).astype(np.float32)[:, None, None] * (1 + noise)
+    vel = vel.astype(dtype)
+    del noise
+    ext = (n - 1) * h
+    sm = sw.SpaceModel(bounding_box=(0, ext) * 3, grid_spacing=(h, h, h),
+                       velocity_model=vel, space_order=8, dtype=dtype)
+    sm.config_boundary(damping_length=500.0,
+                       boundary_condition=("null_neumann",) + ("null_dirichlet",) * 5,
+                       damping_polynomial_degree=3, damping_alpha=0.001)
+    tm = sw.TimeModel(space_model=sm, tf=1.5, saving_stride=0)
+    src = sw.Source(sm, coordinates=[(100.0, ext / 2, ext / 2)])
+    g = np.linspace(0.0, ext, 128)
+    rec = sw.Receiver(sm, coordinates=[(25.0, x, y) for x in g for y in g])
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(15.0, tm))
+
+
+def readme_2d(dtype=np.float32):
+    """The reference README example: 512^2 -> 513^2, so4, 512 receivers."""
+    import simwave_amd as sw
+    vel = np.full((512, 512), 1500.0, dtype=dtype)
+    vel[100:] = 2000.0
+    sm = sw.SpaceModel(bounding_box=(0, 5120, 0, 5120), grid_spacing=(10, 10),
+                       velocity_model=vel, space_order=4, dtype=dtype)
+    sm.config_boundary(damping_length=0,
+                       boundary_condition=("null_neumann", "null_dirichlet",
+                                           "none", "null_dirichlet"))
+    tm = sw.TimeModel(space_model=sm, tf=1.0, saving_stride=0)
+    src = sw.Source(sm, coordinates=[(2560, 2560)], window_radius=1)
+    rec = sw.Receiver(sm, coordinates=[(2560, i) for i in range(0, 5120, 10)],
+                      window_radius=1)
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(10.0, tm))
+
+
+WORKLOADS = {
+    "3d_cd_so8_384": (headline_3d,
+                      "3D constant density so8 384^3 (472^3 extended), "
+                      "12.5 m, 40-pt cubic damping, tf 1.5 s, 128x128 receivers"),
+    "2d_cd_so4_readme": (readme_2d,
+                         "2D constant density so4 512^2 (README example)"),
+}
+
+
+# ---------------------------------------------------------------- baselines
+def _cpu_library(variant):
+    ref = os.path.join(HERE, "oracle", "_ref", "libsw_ref_%s_omp.so" % variant)
+    if os.path.exists(ref):
+        return ref, "reference"
+    port = os.path.join(HERE, "oracle", "_build", "libsw_oracle_%s_omp.so" % variant)
+    if os.path.exists(port):
+        return port, "port"
+    return None, None
+
+
+def time_cpu(kwargs, variant, target_s=15.0):
+    from simwave_amd.middleware import Middleware
+    path, kind = _cpu_library(variant)
+    if path is None:
+        return None
+    lib = ctypes.CDLL(path)
+    fn = lib.forward
+    fn.restype = ctypes.c_double
+    r = kwargs["space_order"] // 2
+    pts = 1
+    for n in kwargs["velocity_model"].shape:
+        pts *= n - 2 * r
+
+    def run(nsteps):
+        kw = dict(kwargs)
+        kw["u_full"] = np.zeros((3,) + kwargs["velocity_model"].shape,
+                                dtype=kwargs["velocity_model"].dtype)
+        kw["shot_record"] = np.zeros_like(kwargs["shot_record"])
+        kw["saving_stride"] = 0
+        kw["num_snapshots"] = 3
+        kw["begin_timestep"] = 1
+        kw["end_timestep"] = nsteps
+        argtypes, args = Middleware.ordered(kw)
+        fn.argtypes = argtypes
+        return fn(*args)
+
+    probe = 2
+    t_probe = run(probe)
+    nsteps = int(max(probe, min(kwargs["end_timestep"],
+                                target_s / max(t_probe / probe, 1e-6))))
+    secs = run(nsteps)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(pts * nsteps / secs / 1e9, 4), "unit": "Gpts/s",
+            "cores": threads, "kind": kind,
+            "sample": "%d of %d time steps (cpu_openmp forward, %d OpenMP threads)"
+                      % (nsteps, kwargs["end_timestep"], threads),
+            "seconds": round(secs, 3)}
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(HERE, "profiles", "%s_%s_traffic.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+# --------------------------------------------------------------------- main
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="3d_cd_so8_384", choices=sorted(WORKLOADS))
+    ap.add_argument("--dtype", default="f32", choices=("f32", "f64"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(shots shard across GPUs with no exchange; run one process "
+              "per GPU).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import simwave_amd as sw
+    from simwave_amd.device import DeviceForward, FLAG_TIME_STEPS
+
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    dtype = np.float32 if args.dtype == "f32" else np.float64
+    builder, desc = WORKLOADS[args.workload]
+    t0 = time.time()
+    parts = builder(dtype=dtype)
+    solver = sw.Solver(**parts)
+    dev = DeviceForward(solver)
+    t_host = time.time() - t0
+    kw = dev.kwargs
+    steps_per_shot = kw["end_timestep"] - kw["begin_timestep"] + 1
+    updates = dev.updates_per_shot
+    log("workload %s: grid %s, %d time steps, %.3e updates/shot, host prep %.1fs"
+        % (args.workload, kw["velocity_model"].shape, steps_per_shot, updates, t_host))
+
+    for _ in range(args.warmup):
+        dev.reset()
+        dev.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dev.reset()
+        dev.run()
+        log("shot %d done" % i)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    gpts = updates / (elapsed / args.steps) / 1e9
+    fused = dev.timings()[6] == 1.0
+
+    roof = None
+    if not args.no_roofline:
+        dev.reset()
+        dev.run(flags=FLAG_TIME_STEPS)
+        tm = dev.timings()
+        avg = tm[4] / max(tm[5], 1)
+        r = kw["space_order"] // 2
+        pts = updates // steps_per_shot
+        bpp = (20 if dtype == np.float32 else 40) + (
+            (4 if dtype == np.float32 else 8) if kw.get("density_model") is not None else 0)
+        achieved = pts * bpp / avg / 1e9
+        kname = "k_step%dd" % kw["velocity_model"].ndim
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(kname, args.workload),
+                "kernel": kname, "avg_launch_us": round(avg * 1e6, 2),
+                "launches": int(tm[5]), "algorithmic_bytes_per_launch": pts * bpp}
+        log("step kernel: %d launches, avg %.1f us, %.1f GB/s algorithmic"
+            % (tm[5], avg * 1e6, achieved))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = time_cpu(kw, dev.variant, args.cpu_seconds)
+        if cpu:
+            log("cpu baseline: %s" % cpu)
+
+    out = {
+        "metric": METRIC,
+        "value": round(gpts, 3),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc,
+                   "grid": list(kw["velocity_model"].shape),
+                   "time_steps": steps_per_shot, "updates_per_shot": updates,
+                   "space_order": kw["space_order"],
+                   "receivers": kw["num_receivers"], "parallelism": "none",
+                   "kernel_path": "fused" if fused else "generic"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,874 @@
+// Host orchestration and C-ABI of the HIP forward operator for gfx950.
+//
+// This translation unit is compiled once per reference variant:
+//   -DSW_DIM=2|3  -DSW_VD=0|1  [-DSW_DOUBLE]
+// and each build exports the reference FFI symbol `forward` with that
+// variant's exact signature (simwave/kernel/backend/middleware.py:149-154
+// binds it by that name), plus suffixed names and a device-pointer entry.
+//
+// Host-side emulation of the reference time loop
+// (simwave/kernel/backend/c_code/forward/constant_density/3d/wave.c:110-618):
+// the same prev/cur/next slot indices are computed on the host for every step,
+// and each logical slot of the caller's `u` array is backed by a device buffer
+// while it is in use.  A slot is uploaded before its first use and downloaded
+// after its last use, so a saving_stride run of any length needs only a few
+// slots of HBM, and the even-stride content exchange becomes a swap of the two
+// device pointers.
+#include "sw_kernels.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#define SW_STR2(x) #x
+#define SW_STR(x) SW_STR2(x)
+#if SW_VD
+#define SW_DENS vd
+#else
+#define SW_DENS cd
+#endif
+#if defined(SW_DOUBLE)
+#define SW_PREC f64
+#else
+#define SW_PREC f32
+#endif
+#define SW_CAT5(a, b, c, d, e) a##b##c##d##e
+#define SW_NAME5(a, b, c, d, e) SW_CAT5(a, b, c, d, e)
+#define SW_VARIANT_STR SW_STR(SW_NAME5(SW_DIM, d_, SW_DENS, _, SW_PREC))
+#define SW_SUFFIX(base) SW_NAME5(base, SW_DIM, d_, SW_DENS, SW_NAME5(_, SW_PREC, , , ))
+
+#define SW_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+using sw::Box;
+using sw::Params;
+
+thread_local std::string g_error;
+thread_local double g_timings[8];
+
+struct Failure {
+    std::string msg;
+};
+
+[[noreturn]] void fail(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Failure{buf};
+}
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess)
+        fail("%s: %s", what, hipGetErrorString(e));
+}
+
+double now_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr unsigned kFlagTimeSteps = 1u;    // accumulate per-launch step time
+constexpr unsigned kFlagGeneric = 2u;      // force the generic kernels
+constexpr int kMaxFusedSources = 64;
+
+// All arguments of one forward call, normalised across variants.
+struct Args {
+    real_t *u;
+    const real_t *vel, *rho, *damp, *wavelet;
+    size_t wavelet_size, wavelet_count;
+    const real_t *c2, *c1;
+    const size_t *bc;
+    const size_t *sint;
+    size_t sint_size;
+    const real_t *svals;
+    size_t svals_size;
+    const size_t *soff;
+    const size_t *rint;
+    size_t rint_size;
+    const real_t *rvals;
+    size_t rvals_size;
+    const size_t *roff;
+    real_t *receivers;
+    size_t nsrc, nrec;
+    size_t nz, nx, ny;
+    real_t dz, dx, dy;
+    size_t stride;
+    real_t dt;
+    size_t n0, n1;
+    size_t space_order, num_snapshots;
+};
+
+// Slot indices of one step and whether the even-stride exchange follows it.
+struct StepSlots {
+    size_t prev, cur, next;
+    bool swap_after;
+    size_t swap_a, swap_b;
+};
+
+// Replays the reference's slot bookkeeping (wave.c:112-124, 569-618).
+std::vector<StepSlots> plan_slots(const Args &a)
+{
+    std::vector<StepSlots> out;
+    if (a.n1 < a.n0)
+        return out;
+    out.reserve(a.n1 - a.n0 + 1);
+    size_t prev = 0, cur = 1, next = 2;
+    for (size_t n = a.n0; n <= a.n1; n++) {
+        if (a.stride == 0) {
+            prev = (n - 1) % 3; cur = n % 3; next = (n + 1) % 3;
+        } else if (a.stride == 1) {
+            prev = n - 1; cur = n; next = n + 1;
+        }
+        StepSlots s{prev, cur, next, false, 0, 0};
+        if (a.stride > 1) {
+            if (n % a.stride == 1) {
+                prev = cur;
+                cur += 1;
+                next += 1;
+                if (a.stride % 2 == 0 && n < a.n1) {
+                    std::swap(cur, next);
+                    s.swap_after = true;
+                    s.swap_a = cur;
+                    s.swap_b = next;
+                }
+            } else {
+                prev = cur;
+                cur = next;
+                next = prev;
+            }
+        }
+        out.push_back(s);
+    }
+    return out;
+}
+
+// Converts one interval/value/offset description (source.py:123-159 layout)
+// into device boxes, validating every index against the grid.
+std::vector<Box> make_boxes(const char *what, size_t count, const size_t *iv,
+                            size_t iv_size, const size_t *off, size_t vsize,
+                            const size_t dims[3])
+{
+    const int D = SW_DIM;
+    std::vector<Box> boxes(count);
+    if (count == 0)
+        return boxes;
+    if (!iv || !off)
+        fail("%s: interval/offset arrays are NULL", what);
+    if (iv_size < count * 2 * D)
+        fail("%s: interval array holds %zu entries, %zu needed", what, iv_size,
+             count * 2 * D);
+    for (size_t s = 0; s < count; s++) {
+        Box b{};
+        size_t nvals = 0;
+        for (int a = 0; a < D; a++) {
+            const size_t lo = iv[s * 2 * D + 2 * a], hi = iv[s * 2 * D + 2 * a + 1];
+            if (lo > hi || hi >= dims[a])
+                fail("%s %zu: axis %d interval [%zu, %zu] outside [0, %zu)",
+                     what, s, a, lo, hi, dims[a]);
+            b.lo[a] = (int)lo;
+            b.hi[a] = (int)hi;
+            nvals += hi - lo + 1;
+        }
+        if (off[s] + nvals > vsize || off[s] > (size_t)0x7fffffff)
+            fail("%s %zu: weights [%zu, %zu) exceed the %zu values", what, s,
+                 off[s], off[s] + nvals, vsize);
+        b.woff = (int)off[s];
+        boxes[s] = b;
+    }
+    return boxes;
+}
+
+// Stream-ordered device allocations released when the object dies.
+struct DeviceArena {
+    hipStream_t stream;
+    std::vector<void *> ptrs;
+    explicit DeviceArena(hipStream_t s) : stream(s) {}
+    template <typename T>
+    T *alloc(size_t count)
+    {
+        if (count == 0)
+            return nullptr;
+        void *p = nullptr;
+        hip_check(hipMallocAsync(&p, count * sizeof(T), stream), "hipMallocAsync");
+        ptrs.push_back(p);
+        return static_cast<T *>(p);
+    }
+    void release(void *p)
+    {
+        auto it = std::find(ptrs.begin(), ptrs.end(), p);
+        if (it != ptrs.end()) {
+            (void)hipFreeAsync(p, stream);
+            ptrs.erase(it);
+        }
+    }
+    ~DeviceArena()
+    {
+        for (void *p : ptrs)
+            (void)hipFreeAsync(p, stream);
+        (void)hipStreamSynchronize(stream);
+    }
+};
+
+void keep_pool_memory()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess)
+            return;
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+    });
+}
+
+// Everything the time loop needs on the device, independent of the slots.
+struct Prepared {
+    Params P{};
+    const Box *rbox = nullptr;
+    const real_t *rw = nullptr;
+    bool fused = false;
+    bool halo_sources = false;   // some source box reaches the halo
+    int r = 0;
+    long long npts = 0;
+};
+
+bool env_flag(const char *name)
+{
+    const char *v = getenv(name);
+    return v && *v && strcmp(v, "0") != 0;
+}
+
+int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? atoi(v) : dflt;
+}
+
+// Validates the call and uploads the run constants.  `dev_models` tells
+// whether vel/rho/damp are device pointers (device entry) or host pointers.
+Prepared prepare(const Args &a, bool dev_models, unsigned flags,
+                 DeviceArena &arena, hipStream_t st)
+{
+    Prepared pr;
+    const size_t so = a.space_order;
+    if (so < 2 || so > 20 || so % 2)
+        fail("space_order %zu: even orders 2..20 are supported", so);
+    const int r = (int)(so / 2);
+    pr.r = r;
+    const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : 1};
+    for (int d = 0; d < SW_DIM; d++)
+        if (dims[d] < (size_t)(2 * r) || dims[d] > (size_t)0x7fffffff)
+            fail("axis %d has %zu points; the stencil radius %d needs >= %d",
+                 d, dims[d], r, 2 * r);
+    if (a.n0 < 1 && a.n1 >= a.n0)
+        fail("begin_timestep must be >= 1 (got %zu)", a.n0);
+    if (a.n1 >= a.n0) {
+        if (a.wavelet_count > 1 && a.wavelet_count != a.nsrc)
+            fail("wavelet_count %zu != num_sources %zu", a.wavelet_count, a.nsrc);
+        if (a.n1 > a.wavelet_size)
+            fail("end_timestep %zu exceeds wavelet_size %zu (the shot record "
+                 "has wavelet_size rows)", a.n1, a.wavelet_size);
+    }
+    if (!a.u || !a.vel || !a.damp || !a.c2 || !a.bc)
+        fail("NULL array argument");
+#if SW_VD
+    if (!a.rho || !a.c1)
+        fail("NULL density or first-order coefficients");
+#endif
+    if (a.nrec && !a.receivers)
+        fail("NULL receivers array");
+    if (a.nsrc && !a.wavelet)
+        fail("NULL wavelet array");
+
+    const long long npts = (long long)dims[0] * (long long)dims[1] * (long long)dims[2];
+    pr.npts = npts;
+    Params &P = pr.P;
+    P.nz = (int)a.nz;
+    P.nx = (int)a.nx;
+    P.ny = SW_DIM == 3 ? (int)a.ny : 1;
+    P.r = r;
+#if SW_DIM == 3
+    P.sz = (long long)a.nx * (long long)a.ny;
+    P.sx = (long long)a.ny;
+#else
+    P.sz = (long long)a.nx;
+    P.sx = 1;
+#endif
+    for (int f = 0; f < 2 * SW_DIM; f++) {
+        const size_t c = a.bc[f];
+        P.bc[f] = (c == 1 || c == 2) ? (int)c : 0;
+    }
+    P.dz2 = a.dz * a.dz;
+    P.dx2 = a.dx * a.dx;
+    P.dy2 = a.dy * a.dy;
+    for (int m = 0; m <= r; m++) {
+        P.c2[m] = a.c2[m];
+        P.c1[m] = SW_VD ? a.c1[m] : (real_t)0;
+    }
+
+    // per-point coefficients: upload v and damp into the q/e buffers and
+    // convert in place (k_prepare reads a point before writing it)
+    real_t *q = arena.alloc<real_t>(npts);
+    real_t *e = arena.alloc<real_t>(npts);
+    const hipMemcpyKind mk = dev_models ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    hip_check(hipMemcpyAsync(q, a.vel, npts * sizeof(real_t), mk, st), "copy velocity");
+    hip_check(hipMemcpyAsync(e, a.damp, npts * sizeof(real_t), mk, st), "copy damp");
+    {
+        const real_t dt = a.dt, dt2 = a.dt * a.dt;
+        const int blocks = (int)std::min<long long>((npts + 255) / 256, 8192);
+        hipLaunchKernelGGL(sw::k_prepare, dim3(blocks), dim3(256), 0, st, q, e, q, e,
+                           npts, dt, dt2);
+        hip_check(hipGetLastError(), "k_prepare launch");
+    }
+    P.q = q;
+    P.e = e;
+#if SW_VD
+    if (dev_models) {
+        P.rho = a.rho;
+    } else {
+        real_t *rho = arena.alloc<real_t>(npts);
+        hip_check(hipMemcpyAsync(rho, a.rho, npts * sizeof(real_t),
+                                 hipMemcpyHostToDevice, st), "copy density");
+        P.rho = rho;
+    }
+#else
+    P.rho = nullptr;
+#endif
+
+    // sources
+    std::vector<Box> sb = make_boxes("source", a.nsrc, a.sint, a.sint_size,
+                                     a.soff, a.svals_size, dims);
+    std::vector<Box> rb = make_boxes("receiver", a.nrec, a.rint, a.rint_size,
+                                     a.roff, a.rvals_size, dims);
+    P.nsrc = (int)a.nsrc;
+    P.wav_multi = a.wavelet_count > 1;
+    if (a.nsrc) {
+        Box *d_sb = arena.alloc<Box>(a.nsrc);
+        real_t *d_sw = arena.alloc<real_t>(std::max<size_t>(a.svals_size, 1));
+        const size_t wlen = a.wavelet_size * std::max<size_t>(a.wavelet_count, 1);
+        real_t *d_wav = arena.alloc<real_t>(std::max<size_t>(wlen, 1));
+        hip_check(hipMemcpyAsync(d_sb, sb.data(), a.nsrc * sizeof(Box),
+                                 hipMemcpyHostToDevice, st), "copy source boxes");
+        if (a.svals_size)
+            hip_check(hipMemcpyAsync(d_sw, a.svals, a.svals_size * sizeof(real_t),
+                                     hipMemcpyHostToDevice, st), "copy source weights");
+        if (wlen)
+            hip_check(hipMemcpyAsync(d_wav, a.wavelet, wlen * sizeof(real_t),
+                                     hipMemcpyHostToDevice, st), "copy wavelet");
+        P.sbox = d_sb;
+        P.sw = d_sw;
+        P.wavelet = d_wav;
+        for (const Box &b : sb) {
+            for (int d = 0; d < SW_DIM; d++)
+                if (b.lo[d] < r || b.hi[d] >= (int)dims[d] - r)
+                    pr.halo_sources = true;
+        }
+    }
+    if (a.nrec) {
+        Box *d_rb = arena.alloc<Box>(a.nrec);
+        real_t *d_rw = arena.alloc<real_t>(std::max<size_t>(a.rvals_size, 1));
+        hip_check(hipMemcpyAsync(d_rb, rb.data(), a.nrec * sizeof(Box),
+                                 hipMemcpyHostToDevice, st), "copy receiver boxes");
+        if (a.rvals_size)
+            hip_check(hipMemcpyAsync(d_rw, a.rvals, a.rvals_size * sizeof(real_t),
+                                     hipMemcpyHostToDevice, st), "copy receiver weights");
+        pr.rbox = d_rb;
+        pr.rw = d_rw;
+    }
+
+    // fused-kernel preconditions: interior of every axis at least r+1 thick
+    // (Neumann mirrors read interior points only), few sources, and for 3D
+    // variable density the middle-axis quirk stride nx equal to ny.
+    bool fused = !(flags & kFlagGeneric) && !env_flag("SIMWAVE_HIP_GENERIC");
+    for (int d = 0; d < SW_DIM; d++)
+        if (dims[d] < (size_t)(3 * r + 1))
+            fused = false;
+    if (a.nsrc > (size_t)kMaxFusedSources)
+        fused = false;
+#if SW_DIM == 3 && SW_VD
+    if (a.nx != a.ny)
+        fused = false;
+#endif
+    pr.fused = fused;
+    return pr;
+}
+
+// ------------------------------------------------------------ launches
+
+#if SW_DIM == 3
+constexpr int kTX = 8;
+
+template <int R>
+void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *un,
+                  int n, hipStream_t st)
+{
+    const int Iy = P.ny - 2 * R, Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    const int gx = (Iy + 63) / 64, gy = (Ix + kTX - 1) / kTX;
+    int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
+    if (zlen <= 0) {
+        const long long cols = (long long)gx * gy;
+        int nzc = (int)std::max<long long>(1, (2048 + cols - 1) / cols);
+        nzc = std::min(nzc, std::max(1, Iz / 16));
+        zlen = (Iz + nzc - 1) / nzc;
+    }
+    const int gz = (Iz + zlen - 1) / zlen;
+    hipLaunchKernelGGL((sw::k_step3d<R, kTX>), dim3(gx, gy, gz), dim3(64, kTX), 0, st,
+                       P, uc, up, un, n, zlen);
+}
+#else
+constexpr int kTB = 256;
+
+template <int R>
+void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *un,
+                  int n, hipStream_t st)
+{
+    const int Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    const int gx = (Ix + kTB - 1) / kTB;
+    int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
+    if (zlen <= 0) {
+        int nzc = std::max(1, (1024 + gx - 1) / gx);
+        nzc = std::min(nzc, std::max(1, Iz / 8));
+        zlen = (Iz + nzc - 1) / nzc;
+    }
+    const int gz = (Iz + zlen - 1) / zlen;
+    hipLaunchKernelGGL((sw::k_step2d<R, kTB>), dim3(gx, gz), dim3(kTB), 0, st,
+                       P, uc, up, un, n, zlen);
+}
+#endif
+
+template <int R>
+void launch_generic(const Params &P, const real_t *uc, const real_t *up, real_t *un,
+                    hipStream_t st)
+{
+#if SW_DIM == 3
+    const long long cnt = (long long)(P.ny - 2 * R) * (P.nx - 2 * R) * (P.nz - 2 * R);
+#else
+    const long long cnt = (long long)(P.nx - 2 * R) * (P.nz - 2 * R);
+#endif
+    if (cnt <= 0)
+        return;
+    hipLaunchKernelGGL((sw::k_stencil_generic<R>), dim3((unsigned)((cnt + 255) / 256)),
+                       dim3(256), 0, st, P, uc, up, un);
+}
+
+using StepFn = void (*)(const Params &, const real_t *, const real_t *, real_t *,
+                        int, hipStream_t);
+using GenFn = void (*)(const Params &, const real_t *, const real_t *, real_t *,
+                       hipStream_t);
+
+template <int... Rs>
+struct Tables {
+    static constexpr StepFn fused[] = {nullptr, &launch_fused<Rs>...};
+    static constexpr GenFn generic[] = {nullptr, &launch_generic<Rs>...};
+};
+using RTable = Tables<1, 2, 3, 4, 5, 6, 7, 8, 9, 10>;
+
+void launch_bc_phases(const Params &P, real_t *un, hipStream_t st)
+{
+    const long long r = P.r;
+#if SW_DIM == 3
+    const long long Iy = P.ny - 2 * r, Ix = P.nx - 2 * r, Iz = P.nz - 2 * r;
+    struct { int axis; int b, a; long long lines; } ph[3] = {
+        {2, P.bc[4], P.bc[5], Iz * Ix},
+        {1, P.bc[2], P.bc[3], Iz * Iy},
+        {0, P.bc[0], P.bc[1], Ix * Iy},
+    };
+    const int nph = 3;
+#else
+    const long long Ix = P.nx - 2 * r, Iz = P.nz - 2 * r;
+    struct { int axis; int b, a; long long lines; } ph[2] = {
+        {2, P.bc[2], P.bc[3], Iz},
+        {0, P.bc[0], P.bc[1], Ix},
+    };
+    const int nph = 2;
+#endif
+    for (int k = 0; k < nph; k++) {
+        if ((!ph[k].b && !ph[k].a) || ph[k].lines <= 0)
+            continue;
+        hipLaunchKernelGGL(sw::k_bc_phase, dim3((unsigned)((ph[k].lines + 255) / 256)),
+                           dim3(256), 0, st, P, un, ph[k].axis);
+    }
+}
+
+// Runs steps [a.n0, a.n1].  `slot_ptr(i)` returns the device buffer backing
+// logical slot i; `before_step`/`after_step` manage residency (host entry)
+// and `swap` exchanges two logical slots.
+template <typename SlotPtr, typename Before, typename After, typename Swap>
+void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &plan,
+               real_t *rec_dev, size_t rec_row0, unsigned flags, hipStream_t st,
+               SlotPtr slot_ptr, Before before_step, After after_step, Swap swap)
+{
+    const Params &P = pr.P;
+    const int r = pr.r;
+    StepFn fused = RTable::fused[r];
+    GenFn generic = RTable::generic[r];
+    const bool timed = flags & kFlagTimeSteps;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double step_ms = 0.0;
+    long long step_launches = 0;
+    if (timed) {
+        hip_check(hipEventCreate(&ev0), "hipEventCreate");
+        hip_check(hipEventCreate(&ev1), "hipEventCreate");
+    }
+    const unsigned rec_blocks = (unsigned)((a.nrec + 255) / 256);
+    for (size_t idx = 0; idx < plan.size(); idx++) {
+        const size_t n = a.n0 + idx;
+        const StepSlots &s = plan[idx];
+        before_step(idx);
+        const real_t *up = slot_ptr(s.prev);
+        const real_t *uc = slot_ptr(s.cur);
+        real_t *un = slot_ptr(s.next);
+        if (pr.fused) {
+            if (pr.halo_sources)
+                hipLaunchKernelGGL(sw::k_inject, dim3(1), dim3(256), 0, st, P, un,
+                                   (int)n, 1);
+            if (timed)
+                hip_check(hipEventRecord(ev0, st), "hipEventRecord");
+            fused(P, uc, up, un, (int)n, st);
+            if (timed)
+                hip_check(hipEventRecord(ev1, st), "hipEventRecord");
+        } else {
+            if (timed)
+                hip_check(hipEventRecord(ev0, st), "hipEventRecord");
+            generic(P, uc, up, un, st);
+            if (timed)
+                hip_check(hipEventRecord(ev1, st), "hipEventRecord");
+            if (P.nsrc)
+                hipLaunchKernelGGL(sw::k_inject, dim3(1), dim3(256), 0, st, P, un,
+                                   (int)n, 0);
+            launch_bc_phases(P, un, st);
+        }
+        if (a.nrec)
+            hipLaunchKernelGGL(sw::k_receivers, dim3(rec_blocks), dim3(256), 0, st, P,
+                               uc, pr.rbox, pr.rw, (int)a.nrec,
+                               rec_dev + (n - 1 - rec_row0) * a.nrec);
+        hip_check(hipGetLastError(), "step launch");
+        if (timed) {
+            float ms = 0.f;
+            hip_check(hipEventSynchronize(ev1), "hipEventSynchronize");
+            hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+            step_ms += ms;
+            step_launches++;
+        }
+        if (s.swap_after)
+            swap(s.swap_a, s.swap_b);
+        after_step(idx);
+    }
+    if (timed) {
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+        g_timings[4] = step_ms * 1e-3;
+        g_timings[5] = (double)step_launches;
+    }
+}
+
+// --------------------------------------------------------- host entry
+// Arrays are host pointers (the reference contract).  Returns wall seconds.
+double run_host(const Args &a)
+{
+    const double t0 = now_s();
+    for (double &t : g_timings)
+        t = 0.0;
+    int ndev = 0;
+    hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    if (ndev < 1)
+        fail("no HIP device visible");
+    keep_pool_memory();
+    hipStream_t st = nullptr;
+    hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+
+    std::vector<StepSlots> plan = plan_slots(a);
+    const size_t D = SW_DIM;
+    (void)D;
+    // slot residency: first and last step index of every logical slot
+    std::vector<long long> first(a.num_snapshots, -1), last(a.num_snapshots, -1);
+    auto touch = [&](size_t slot, size_t idx) {
+        if (slot >= a.num_snapshots)
+            fail("slot %zu used by step %zu is outside the %zu snapshots", slot,
+                 a.n0 + idx, a.num_snapshots);
+        if (first[slot] < 0)
+            first[slot] = (long long)idx;
+        last[slot] = (long long)idx;
+    };
+    for (size_t idx = 0; idx < plan.size(); idx++) {
+        touch(plan[idx].prev, idx);
+        touch(plan[idx].cur, idx);
+        touch(plan[idx].next, idx);
+        if (plan[idx].swap_after) {
+            touch(plan[idx].swap_a, idx);
+            touch(plan[idx].swap_b, idx);
+        }
+    }
+
+    double t_prep = 0, t_loop = 0, t_down = 0;
+    {
+        DeviceArena arena(st);
+        Prepared pr = prepare(a, false, 0, arena, st);
+        const size_t npts = (size_t)pr.npts;
+        const size_t slot_bytes = npts * sizeof(real_t);
+
+        // receivers rows n0-1 .. n1-1 are fully written by the run
+        const size_t rows = plan.size();
+        real_t *rec_dev = arena.alloc<real_t>(rows * a.nrec);
+
+        std::vector<real_t *> phys(a.num_snapshots, nullptr);
+        std::vector<real_t *> pool;
+        auto acquire = [&]() -> real_t * {
+            if (!pool.empty()) {
+                real_t *p = pool.back();
+                pool.pop_back();
+                return p;
+            }
+            return arena.alloc<real_t>(npts);
+        };
+        // bucket slots by the step that uploads / downloads them
+        std::vector<std::vector<size_t>> up_at(plan.size()), down_at(plan.size());
+        for (size_t sl = 0; sl < a.num_snapshots; sl++) {
+            if (first[sl] < 0)
+                continue;
+            up_at[first[sl]].push_back(sl);
+            down_at[last[sl]].push_back(sl);
+        }
+        hip_check(hipStreamSynchronize(st), "prepare");
+        t_prep = now_s() - t0;
+        const double tl0 = now_s();
+
+        auto before = [&](size_t idx) {
+            for (size_t sl : up_at[idx]) {
+                real_t *d = acquire();
+                hip_check(hipMemcpyAsync(d, a.u + sl * npts, slot_bytes,
+                                         hipMemcpyHostToDevice, st), "upload slot");
+                phys[sl] = d;
+            }
+        };
+        auto after = [&](size_t idx) {
+            for (size_t sl : down_at[idx]) {
+                hip_check(hipMemcpyAsync(a.u + sl * npts, phys[sl], slot_bytes,
+                                         hipMemcpyDeviceToHost, st), "download slot");
+                pool.push_back(phys[sl]);
+                phys[sl] = nullptr;
+            }
+        };
+        auto slot_ptr = [&](size_t sl) { return phys[sl]; };
+        auto swap = [&](size_t x, size_t y) { std::swap(phys[x], phys[y]); };
+
+        time_loop(a, pr, plan, rec_dev, a.n0 - 1, 0u, st, slot_ptr, before, after, swap);
+        hip_check(hipStreamSynchronize(st), "time loop");
+        const double tl1 = now_s();
+        t_loop = tl1 - tl0;
+        if (rows && a.nrec)
+            hip_check(hipMemcpyAsync(a.receivers + (a.n0 - 1) * a.nrec, rec_dev,
+                                     rows * a.nrec * sizeof(real_t),
+                                     hipMemcpyDeviceToHost, st), "download receivers");
+        hip_check(hipStreamSynchronize(st), "download");
+        t_down = now_s() - tl1;
+        g_timings[6] = pr.fused ? 1.0 : 0.0;
+    }
+    const double total = now_s() - t0;
+    g_timings[0] = total;
+    g_timings[1] = t_prep;
+    g_timings[2] = t_loop;
+    g_timings[3] = t_down;
+    return total;
+}
+
+// -------------------------------------------------------- device entry
+// u, velocity, density, damp and receivers are device pointers; u holds all
+// num_snapshots slots, receivers wavelet_size rows.  Everything else is host
+// memory.  Runs on `stream` (nullptr: the null stream) and returns after the
+// stream has drained; the return value is wall seconds of the call.
+double run_device(const Args &a, hipStream_t st, unsigned flags)
+{
+    const double t0 = now_s();
+    for (double &t : g_timings)
+        t = 0.0;
+    keep_pool_memory();
+    std::vector<StepSlots> plan = plan_slots(a);
+    for (size_t idx = 0; idx < plan.size(); idx++) {
+        const StepSlots &s = plan[idx];
+        size_t m = std::max({s.prev, s.cur, s.next});
+        if (s.swap_after)
+            m = std::max({m, s.swap_a, s.swap_b});
+        if (m >= a.num_snapshots)
+            fail("slot %zu used by step %zu is outside the %zu snapshots", m,
+                 a.n0 + idx, a.num_snapshots);
+    }
+    DeviceArena arena(st);
+    Prepared pr = prepare(a, true, flags, arena, st);
+    const size_t npts = (size_t)pr.npts;
+    const double tl0 = now_s();
+    auto slot_ptr = [&](size_t sl) { return a.u + sl * npts; };
+    auto nop = [](size_t) {};
+    auto swap = [&](size_t x, size_t y) {
+        const int blocks = (int)std::min<long long>((long long)(npts + 255) / 256, 16384);
+        hipLaunchKernelGGL(sw::k_swap, dim3(blocks), dim3(256), 0, st,
+                           a.u + x * npts, a.u + y * npts, (long long)npts);
+    };
+    time_loop(a, pr, plan, a.receivers, 0, flags, st, slot_ptr, nop, nop, swap);
+    hip_check(hipStreamSynchronize(st), "time loop");
+    g_timings[2] = now_s() - tl0;
+    g_timings[6] = pr.fused ? 1.0 : 0.0;
+    const double total = now_s() - t0;
+    g_timings[0] = total;
+    return total;
+}
+
+template <typename F>
+double guarded(F &&f)
+{
+    try {
+        g_error.clear();
+        return f();
+    } catch (const Failure &e) {
+        g_error = e.msg;
+    } catch (const std::exception &e) {
+        g_error = std::string("internal error: ") + e.what();
+    } catch (...) {
+        g_error = "internal error";
+    }
+    (void)hipGetLastError();
+    return -1.0;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+// The parameter lists below are the reference's, variant by variant
+// (constant_density/3d/wave.c:23-36, constant_density/2d/wave.c:23-36,
+//  variable_density/3d/wave.c:23-36, variable_density/2d/wave.c:23-36).
+
+#if SW_DIM == 3
+#define SW_GRID_PARAMS size_t nz, size_t nx, size_t ny, real_t dz, real_t dx, real_t dy
+#define SW_GRID_FILL a.nz = nz; a.nx = nx; a.ny = ny; a.dz = dz; a.dx = dx; a.dy = dy;
+#define SW_GRID_ARGS nz, nx, ny, dz, dx, dy
+#else
+#define SW_GRID_PARAMS size_t nz, size_t nx, real_t dz, real_t dx
+#define SW_GRID_FILL a.nz = nz; a.nx = nx; a.ny = 1; a.dz = dz; a.dx = dx; a.dy = dx;
+#define SW_GRID_ARGS nz, nx, dz, dx
+#endif
+
+#if SW_VD
+#define SW_MODEL_PARAMS real_t *u, real_t *velocity, real_t *density, real_t *damp
+#define SW_MODEL_FILL a.u = u; a.vel = velocity; a.rho = density; a.damp = damp;
+#define SW_MODEL_ARGS u, velocity, density, damp
+#define SW_COEFF_PARAMS real_t *coeff_order2, real_t *coeff_order1
+#define SW_COEFF_FILL a.c2 = coeff_order2; a.c1 = coeff_order1;
+#define SW_COEFF_ARGS coeff_order2, coeff_order1
+#else
+#define SW_MODEL_PARAMS real_t *u, real_t *velocity, real_t *damp
+#define SW_MODEL_FILL a.u = u; a.vel = velocity; a.rho = nullptr; a.damp = damp;
+#define SW_MODEL_ARGS u, velocity, damp
+#define SW_COEFF_PARAMS real_t *coeff
+#define SW_COEFF_FILL a.c2 = coeff; a.c1 = nullptr;
+#define SW_COEFF_ARGS coeff
+#endif
+
+#define SW_FORWARD_PARAMS                                                        \
+    SW_MODEL_PARAMS, real_t *wavelet, size_t wavelet_size, size_t wavelet_count, \
+        SW_COEFF_PARAMS, size_t *boundary_conditions, size_t *src_points_interval, \
+        size_t src_points_interval_size, real_t *src_points_values,              \
+        size_t src_points_values_size, size_t *src_points_values_offset,         \
+        size_t *rec_points_interval, size_t rec_points_interval_size,            \
+        real_t *rec_points_values, size_t rec_points_values_size,                \
+        size_t *rec_points_values_offset, real_t *receivers, size_t num_sources, \
+        size_t num_receivers, SW_GRID_PARAMS, size_t saving_stride, real_t dt,   \
+        size_t begin_timestep, size_t end_timestep, size_t space_order,          \
+        size_t num_snapshots
+
+#define SW_FORWARD_ARGS                                                          \
+    SW_MODEL_ARGS, wavelet, wavelet_size, wavelet_count, SW_COEFF_ARGS,          \
+        boundary_conditions, src_points_interval, src_points_interval_size,      \
+        src_points_values, src_points_values_size, src_points_values_offset,     \
+        rec_points_interval, rec_points_interval_size, rec_points_values,        \
+        rec_points_values_size, rec_points_values_offset, receivers,             \
+        num_sources, num_receivers, SW_GRID_ARGS, saving_stride, dt,             \
+        begin_timestep, end_timestep, space_order, num_snapshots
+
+static Args fill_args(SW_FORWARD_PARAMS)
+{
+    Args a{};
+    SW_MODEL_FILL
+    a.wavelet = wavelet;
+    a.wavelet_size = wavelet_size;
+    a.wavelet_count = wavelet_count;
+    SW_COEFF_FILL
+    a.bc = boundary_conditions;
+    a.sint = src_points_interval;
+    a.sint_size = src_points_interval_size;
+    a.svals = src_points_values;
+    a.svals_size = src_points_values_size;
+    a.soff = src_points_values_offset;
+    a.rint = rec_points_interval;
+    a.rint_size = rec_points_interval_size;
+    a.rvals = rec_points_values;
+    a.rvals_size = rec_points_values_size;
+    a.roff = rec_points_values_offset;
+    a.receivers = receivers;
+    a.nsrc = num_sources;
+    a.nrec = num_receivers;
+    SW_GRID_FILL
+    a.stride = saving_stride;
+    a.dt = dt;
+    a.n0 = begin_timestep;
+    a.n1 = end_timestep;
+    a.space_order = space_order;
+    a.num_snapshots = num_snapshots;
+    return a;
+}
+
+// Reference FFI name: one variant per library, like the reference's
+// runtime-compiled wave.c (middleware.py:26-51).
+SW_EXPORT double forward(SW_FORWARD_PARAMS)
+{
+    return guarded([&] { return run_host(fill_args(SW_FORWARD_ARGS)); });
+}
+
+SW_EXPORT double SW_SUFFIX(simwave_hip_forward_)(SW_FORWARD_PARAMS)
+{
+    return guarded([&] { return run_host(fill_args(SW_FORWARD_ARGS)); });
+}
+
+SW_EXPORT double SW_SUFFIX(simwave_hip_forward_device_)(SW_FORWARD_PARAMS, void *stream,
+                                                        unsigned flags)
+{
+    return guarded([&] {
+        return run_device(fill_args(SW_FORWARD_ARGS), (hipStream_t)stream, flags);
+    });
+}
+
+SW_EXPORT const char *simwave_hip_last_error(void)
+{
+    return g_error.c_str();
+}
+
+SW_EXPORT int simwave_hip_last_timings(double *out, int n)
+{
+    const int m = std::min(n, 8);
+    for (int i = 0; i < m; i++)
+        out[i] = g_timings[i];
+    return m;
+}
+
+SW_EXPORT const char *simwave_hip_variant(void)
+{
+    return SW_VARIANT_STR;
+}

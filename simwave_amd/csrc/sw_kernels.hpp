@@ -1,0 +1,744 @@
+// Device kernels of the acoustic forward operator for gfx950 (MI355X).
+//
+// Reference semantics: simwave/kernel/backend/c_code/forward/
+//   {constant,variable}_density/{2d,3d}/wave.c, sections 1-4 of the time loop.
+//
+// Floating-point contract: every kernel reproduces the reference C's types and
+// evaluation order (centre term first, pairs added in radius order, division by
+// h^2, the double-promoted `2.0/den * u` update), and the library is compiled
+// with -ffp-contract=off, so results are bit-identical to the reference C on
+// x86 (IEEE binary32/64, round-to-nearest, denormals kept).
+//
+// Per-point model coefficients are precomputed once per run by k_prepare:
+//   q = dt^2 / s              (the reference's `dtSquared / slowness`)
+//   e = damp*dt / (2 s)       (the reference's damping term)
+// with s = 1/(v*v) rounded to the working type exactly as the C does.  Where
+// e == 0 (the whole interior outside the damping layer) the reference's
+// den/num are exactly 1 and the update reduces to 2u - u_prev + q*L with the
+// same roundings, which the kernels exploit without changing a bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#if defined(SW_DOUBLE)
+typedef double real_t;
+#else
+typedef float real_t;
+#endif
+
+#ifndef SW_DIM
+#error "SW_DIM must be 2 or 3"
+#endif
+#ifndef SW_VD
+#define SW_VD 0
+#endif
+
+namespace sw {
+
+constexpr int kMaxR = 10;
+
+// Inclusive index box of one source or receiver plus the offset of its
+// per-axis weights (z weights, then x, then y) in the weight array.
+struct Box {
+    int lo[3];
+    int hi[3];
+    int woff;
+    int pad;
+};
+
+// Everything a time step needs that does not change between steps.
+struct Params {
+    int nz, nx, ny;          // extended shape; ny == 1 in 2D
+    int r;                   // stencil radius
+    long long sz, sx;        // element strides of z and the middle axis (3D)
+    int bc[6];               // z_b z_a x_b x_a y_b y_a : 0 none, 1 Dirichlet, 2 Neumann
+    real_t dz2, dx2, dy2;    // squared spacings (2D: dx2 is the contiguous axis)
+    real_t c2[kMaxR + 1];    // second-derivative half coefficients
+    real_t c1[kMaxR + 1];    // first-derivative half coefficients (VD)
+    const real_t *q;         // dt^2/s per point
+    const real_t *e;         // damping term per point
+    const real_t *rho;       // density (VD)
+    // sources
+    int nsrc;
+    int wav_multi;           // wavelet_count > 1
+    const Box *sbox;
+    const real_t *sw;        // source weights
+    const real_t *wavelet;
+};
+
+// ---------------------------------------------------------------- helpers
+
+// Leap-frog update with damping; `value` is the spatial operator result
+// (the reference's `value` before `value *= (dt2/s)/den`).
+// Reference: cd-3d wave.c:177-185.
+__device__ __forceinline__ real_t leapfrog(real_t value, real_t q, real_t e,
+                                           real_t uc, real_t up)
+{
+    if (e == (real_t)0) {
+        // den == num == 1: (dt2/s)/den == q, 2.0/den == 2.0, (num/den)*up == up
+        value = value * q;
+        double t = 2.0 * (double)uc - (double)up;
+        t = t + (double)value;
+        return (real_t)t;
+    }
+    const real_t den = (real_t)(1.0 + (double)e);
+    const real_t num = (real_t)(1.0 - (double)e);
+    value = value * (q / den);
+    const double a = 2.0 / (double)den;
+    const real_t b = num / den;
+    double t = a * (double)uc - (double)(b * up);
+    t = t + (double)value;
+    return (real_t)t;
+}
+
+// Source contribution at one point: dt2/s * kws * w / den.
+// Reference: cd-3d wave.c:272-277.
+__device__ __forceinline__ real_t source_term(real_t q, real_t e, real_t kws,
+                                              real_t amp)
+{
+    const real_t den = (real_t)(1.0 + (double)e);
+    return q * kws * amp / den;
+}
+
+__device__ __forceinline__ real_t wavelet_amp(const Params &P, int n, int s)
+{
+    const long long idx = P.wav_multi ? (long long)(n - 1) * P.nsrc + s
+                                      : (long long)(n - 1);
+    return P.wavelet[idx];
+}
+
+// kws weight of box point (i, j[, k]) of box b.
+__device__ __forceinline__ real_t box_weight(const Box &b, const real_t *w,
+                                             int i, int j, int k)
+{
+    const int cz = b.hi[0] - b.lo[0] + 1;
+    const real_t *wz = w + b.woff;
+    const real_t *wx = wz + cz;
+#if SW_DIM == 3
+    const int cx = b.hi[1] - b.lo[1] + 1;
+    const real_t *wy = wx + cx;
+    return wz[i - b.lo[0]] * wx[j - b.lo[1]] * wy[k - b.lo[2]];
+#else
+    (void)k;
+    return wz[i - b.lo[0]] * wx[j - b.lo[1]];
+#endif
+}
+
+__device__ __forceinline__ bool box_has(const Box &b, int i, int j, int k)
+{
+#if SW_DIM == 3
+    return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] &&
+           k >= b.lo[2] && k <= b.hi[2];
+#else
+    (void)k;
+    return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1];
+#endif
+}
+
+// Adds, in source order, every source whose box holds interior point p to
+// the freshly computed value v.  Reference: cd-3d wave.c:208-295 (the
+// sequential C order of the `+=`).
+__device__ __forceinline__ real_t add_sources(const Params &P, int n, int i,
+                                              int j, int k, long long p,
+                                              real_t v)
+{
+    for (int s = 0; s < P.nsrc; s++) {
+        const Box b = P.sbox[s];
+        if (!box_has(b, i, j, k))
+            continue;
+        const real_t amp = wavelet_amp(P, n, s);
+        if (amp == (real_t)0)
+            continue;
+        const real_t kws = box_weight(b, P.sw, i, j, k);
+        v += source_term(P.q[p], P.e[p], kws, amp);
+    }
+    return v;
+}
+
+// ------------------------------------------------------------ k_prepare
+// q = dt2/s and e = damp*dt/(2s) with s = 1/(v*v).  Reference: cd-3d
+// wave.c:177-181 (per-point terms that do not depend on the time step).
+// Called in place (q aliases vel, e aliases damp): each point is read
+// before it is written, by the same thread.
+__global__ void k_prepare(const real_t *vel, const real_t *damp,
+                          real_t *q, real_t *e,
+                          long long npts, real_t dt, real_t dt2)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         p < npts; p += (long long)gridDim.x * blockDim.x) {
+        const real_t v = vel[p];
+        const real_t s = (real_t)(1.0 / (double)(v * v));
+        q[p] = dt2 / s;
+        e[p] = damp[p] * dt / (2 * s);
+    }
+}
+
+// -------------------------------------------------- boundary predicates
+// Value a point carries after the boundary phases that precede a given read,
+// for grids whose interior spans at least r+1 points on every axis (then the
+// Neumann mirrors only read interior points and only write halo points).
+// Reference order: contiguous axis, middle axis (3D), z; each face line
+// applies before-Dirichlet, before-Neumann, after-Dirichlet, after-Neumann.
+// Reference: cd-3d wave.c:304-480, cd-2d wave.c:280-393.
+struct BcFlags {
+    bool zc;  // zeroed by a Dirichlet face of the contiguous axis
+    bool zm;  // ... of the middle axis (3D)
+    bool zz;  // ... of the z axis
+};
+
+// Writes the final value of interior point (i,j,k) and every halo point the
+// Neumann faces copy from it.  v is the post-source value.
+__device__ __forceinline__ void store_with_bc(const Params &P, real_t *un,
+                                              long long p, int i, int j, int k,
+                                              real_t v)
+{
+    const int r = P.r;
+#if SW_DIM == 3
+    const int cb = P.bc[4], ca = P.bc[5], cn = P.ny;  // contiguous axis: y
+    const int mb = P.bc[2], ma = P.bc[3], mn = P.nx;  // middle axis: x
+    const int kc = k, km = j;
+    const long long sc = 1, sm = P.sx;
+#else
+    const int cb = P.bc[2], ca = P.bc[3], cn = P.nx;  // contiguous axis: x
+    const int kc = j;
+    const long long sc = 1;
+#endif
+    const int zb = P.bc[0], za = P.bc[1];
+    const long long sz = P.sz;
+
+    const bool zc = (cb == 1 && kc == r) || (ca == 1 && kc == cn - r - 1);
+#if SW_DIM == 3
+    const bool zm = (mb == 1 && km == r) || (ma == 1 && km == mn - r - 1);
+#else
+    const bool zm = false;
+#endif
+    const bool zz = (zb == 1 && i == r) || (za == 1 && i == P.nz - r - 1);
+    const real_t zero = (real_t)0;
+    un[p] = (zc || zm || zz) ? zero : v;
+
+    // contiguous-axis faces (first phase)
+    if (cb == 2) {
+        const int m = kc - r;
+        if (m >= 1 && m <= r)
+            un[p - 2 * m * sc] = v;
+    }
+    if (ca == 2) {
+        const int m = (cn - r - 1) - kc;
+        if (m >= 1 && m <= r)
+            un[p + 2 * m * sc] = (cb == 1 && kc == r) ? zero : v;
+    }
+#if SW_DIM == 3
+    // middle-axis faces (second phase): reads see contiguous-axis zeroing
+    if (mb == 2) {
+        const int m = km - r;
+        if (m >= 1 && m <= r)
+            un[p - 2 * m * sm] = zc ? zero : v;
+    }
+    if (ma == 2) {
+        const int m = (mn - r - 1) - km;
+        if (m >= 1 && m <= r)
+            un[p + 2 * m * sm] = (zc || (mb == 1 && km == r)) ? zero : v;
+    }
+#endif
+    // z faces (last phase)
+    if (zb == 2) {
+        const int m = i - r;
+        if (m >= 1 && m <= r)
+            un[p - 2 * m * sz] = (zc || zm) ? zero : v;
+    }
+    if (za == 2) {
+        const int m = (P.nz - r - 1) - i;
+        if (m >= 1 && m <= r)
+            un[p + 2 * m * sz] = (zc || zm || (zb == 1 && i == r)) ? zero : v;
+    }
+}
+
+// ------------------------------------------------ fused step, 3D
+// One workgroup = TY (=64, one wavefront row) points along the contiguous
+// axis y x TX rows along x, marching along z over [z0, z0+zlen).  The
+// current plane of u (and rho for VD) lives in LDS with an R-wide halo; the
+// z neighbours live in a per-thread register queue of 2R+1 values.
+#if SW_DIM == 3
+template <int R, int TX>
+__global__ void __launch_bounds__(64 * TX)
+k_step3d(Params P, const real_t *__restrict__ uc, const real_t *up,
+         real_t *un, int n, int zlen)
+{
+    constexpr int TY = 64;
+    constexpr int W = TY + 2 * R;
+    __shared__ real_t tile[TX + 2 * R][W];
+#if SW_VD
+    __shared__ real_t rtile[TX + 2 * R][W];
+#endif
+    const int tx = threadIdx.x;          // along y
+    const int ty = threadIdx.y;          // along x
+    const int k = R + blockIdx.x * TY + tx;
+    const int j = R + blockIdx.y * TX + ty;
+    const int z0 = R + blockIdx.z * zlen;
+    const int z1 = min(z0 + zlen, P.nz - R);
+    const int ny = P.ny, nx = P.nx;
+    const long long sz = P.sz, sx = P.sx;
+    const bool in_arr = (k < ny) && (j < nx);
+    const bool active = (k < ny - R) && (j < nx - R);
+    const long long col = (long long)j * sx + k;
+
+    real_t qz[2 * R + 1];
+#if SW_VD
+    real_t rz[2 * R + 1];
+#endif
+#pragma unroll
+    for (int m = 0; m < 2 * R; m++) {
+        const long long o = (long long)(z0 - R + m) * sz + col;
+        qz[m + 1] = in_arr ? uc[o] : (real_t)0;
+#if SW_VD
+        rz[m + 1] = in_arr ? P.rho[o] : (real_t)0;
+#endif
+    }
+
+    for (int i = z0; i < z1; i++) {
+#pragma unroll
+        for (int m = 0; m < 2 * R; m++) {
+            qz[m] = qz[m + 1];
+#if SW_VD
+            rz[m] = rz[m + 1];
+#endif
+        }
+        {
+            const long long o = (long long)(i + R) * sz + col;
+            qz[2 * R] = in_arr ? uc[o] : (real_t)0;
+#if SW_VD
+            rz[2 * R] = in_arr ? P.rho[o] : (real_t)0;
+#endif
+        }
+        const long long plane = (long long)i * sz;
+        __syncthreads();
+        tile[ty + R][tx + R] = qz[R];
+#if SW_VD
+        rtile[ty + R][tx + R] = rz[R];
+#endif
+        // x halo rows: R rows below and above the tile
+        if (k < ny) {
+            const int j0 = R + (int)blockIdx.y * TX;
+            for (int h = ty; h < R; h += TX) {
+                const int jl = j0 - R + h;          // >= 0
+                const int jh = j0 + TX + h;
+                tile[h][tx + R] = uc[plane + (long long)jl * sx + k];
+#if SW_VD
+                rtile[h][tx + R] = P.rho[plane + (long long)jl * sx + k];
+#endif
+                if (jh < nx) {
+                    tile[TX + R + h][tx + R] = uc[plane + (long long)jh * sx + k];
+#if SW_VD
+                    rtile[TX + R + h][tx + R] = P.rho[plane + (long long)jh * sx + k];
+#endif
+                }
+            }
+        }
+        // y halo columns
+        if (tx < R && j < nx) {
+            const int kl = k - R;                   // >= 0
+            const int kh = blockIdx.x * TY + R + TY + tx;
+            tile[ty + R][tx] = uc[plane + (long long)j * sx + kl];
+#if SW_VD
+            rtile[ty + R][tx] = P.rho[plane + (long long)j * sx + kl];
+#endif
+            if (kh < ny) {
+                tile[ty + R][TY + R + tx] = uc[plane + (long long)j * sx + kh];
+#if SW_VD
+                rtile[ty + R][TY + R + tx] = P.rho[plane + (long long)j * sx + kh];
+#endif
+            }
+        }
+        __syncthreads();
+        if (!active)
+            continue;
+
+        const long long p = plane + col;
+        const real_t c = qz[R];
+        real_t acc_c = P.c2[0] * c;
+        real_t acc_m = P.c2[0] * c;
+        real_t acc_z = P.c2[0] * c;
+#if SW_VD
+        real_t du_c = 0, dr_c = 0, du_m = 0, dr_m = 0, du_z = 0, dr_z = 0;
+#endif
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            const real_t cp = tile[ty + R][tx + R + m], cm = tile[ty + R][tx + R - m];
+            const real_t mp = tile[ty + R + m][tx + R], mm = tile[ty + R - m][tx + R];
+            acc_c += P.c2[m] * (cp + cm);
+            acc_m += P.c2[m] * (mp + mm);
+            acc_z += P.c2[m] * (qz[R + m] + qz[R - m]);
+#if SW_VD
+            du_c += P.c1[m] * (cp - cm);
+            dr_c += P.c1[m] * (rtile[ty + R][tx + R + m] - rtile[ty + R][tx + R - m]);
+            du_m += P.c1[m] * (mp - mm);
+            dr_m += P.c1[m] * (rtile[ty + R + m][tx + R] - rtile[ty + R - m][tx + R]);
+            du_z += P.c1[m] * (qz[R + m] - qz[R - m]);
+            dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+#endif
+        }
+        real_t value = 0;
+        value += acc_c / P.dy2 + acc_m / P.dx2 + acc_z / P.dz2;
+#if SW_VD
+        {
+            const real_t t_c = (du_c * dr_c) / (4 * P.dy2);
+            const real_t t_m = (du_m * dr_m) / (4 * P.dx2);
+            const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
+            value -= (t_c + t_m + t_z) / rz[R];
+        }
+#endif
+        real_t v = leapfrog(value, P.q[p], P.e[p], c, up[p]);
+        if (P.nsrc)
+            v = add_sources(P, n, i, j, k, p, v);
+        store_with_bc(P, un, p, i, j, k, v);
+    }
+}
+#endif
+
+// ------------------------------------------------ fused step, 2D
+// One workgroup = TB points along the contiguous axis x, marching along z.
+#if SW_DIM == 2
+template <int R, int TB>
+__global__ void __launch_bounds__(TB)
+k_step2d(Params P, const real_t *__restrict__ uc, const real_t *up,
+         real_t *un, int n, int zlen)
+{
+    constexpr int W = TB + 2 * R;
+    __shared__ real_t row[W];
+#if SW_VD
+    __shared__ real_t rrow[W];
+#endif
+    const int tx = threadIdx.x;
+    const int j = R + blockIdx.x * TB + tx;
+    const int z0 = R + blockIdx.y * zlen;
+    const int z1 = min(z0 + zlen, P.nz - R);
+    const int nx = P.nx;
+    const long long sz = P.sz;
+    const bool in_arr = j < nx;
+    const bool active = j < nx - R;
+
+    real_t qz[2 * R + 1];
+#if SW_VD
+    real_t rz[2 * R + 1];
+#endif
+#pragma unroll
+    for (int m = 0; m < 2 * R; m++) {
+        const long long o = (long long)(z0 - R + m) * sz + j;
+        qz[m + 1] = in_arr ? uc[o] : (real_t)0;
+#if SW_VD
+        rz[m + 1] = in_arr ? P.rho[o] : (real_t)0;
+#endif
+    }
+    for (int i = z0; i < z1; i++) {
+#pragma unroll
+        for (int m = 0; m < 2 * R; m++) {
+            qz[m] = qz[m + 1];
+#if SW_VD
+            rz[m] = rz[m + 1];
+#endif
+        }
+        {
+            const long long o = (long long)(i + R) * sz + j;
+            qz[2 * R] = in_arr ? uc[o] : (real_t)0;
+#if SW_VD
+            rz[2 * R] = in_arr ? P.rho[o] : (real_t)0;
+#endif
+        }
+        const long long plane = (long long)i * sz;
+        __syncthreads();
+        row[tx + R] = qz[R];
+#if SW_VD
+        rrow[tx + R] = rz[R];
+#endif
+        if (tx < R) {
+            const int jl = j - R;
+            const int jh = blockIdx.x * TB + R + TB + tx;
+            row[tx] = uc[plane + jl];
+#if SW_VD
+            rrow[tx] = P.rho[plane + jl];
+#endif
+            if (jh < nx) {
+                row[TB + R + tx] = uc[plane + jh];
+#if SW_VD
+                rrow[TB + R + tx] = P.rho[plane + jh];
+#endif
+            }
+        }
+        __syncthreads();
+        if (!active)
+            continue;
+        const long long p = plane + j;
+        const real_t c = qz[R];
+        real_t acc_c = P.c2[0] * c;
+        real_t acc_z = P.c2[0] * c;
+#if SW_VD
+        real_t du_c = 0, dr_c = 0, du_z = 0, dr_z = 0;
+#endif
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            const real_t cp = row[tx + R + m], cm = row[tx + R - m];
+            acc_c += P.c2[m] * (cp + cm);
+            acc_z += P.c2[m] * (qz[R + m] + qz[R - m]);
+#if SW_VD
+            du_c += P.c1[m] * (cp - cm);
+            dr_c += P.c1[m] * (rrow[tx + R + m] - rrow[tx + R - m]);
+            du_z += P.c1[m] * (qz[R + m] - qz[R - m]);
+            dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+#endif
+        }
+        real_t value = 0;
+        value += acc_c / P.dx2 + acc_z / P.dz2;
+#if SW_VD
+        {
+            const real_t t_c = (du_c * dr_c) / (4 * P.dx2);
+            const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
+            value -= (t_c + t_z) / rz[R];
+        }
+#endif
+        real_t v = leapfrog(value, P.q[p], P.e[p], c, up[p]);
+        if (P.nsrc)
+            v = add_sources(P, n, i, j, 0, p, v);
+        store_with_bc(P, un, p, i, j, 0, v);
+    }
+}
+#endif
+
+// ------------------------------------------------ generic (exact) path
+// Used where the fused kernel's preconditions do not hold (interior thinner
+// than r+1 points, too many sources, or the 3D variable-density x stride
+// quirk with nx != ny).  Plain global loads, one thread per interior point.
+// Reference: vd-3d wave.c:145-214 including the nx stride of the middle-axis
+// first derivatives (:184-186).
+template <int R>
+__global__ void __launch_bounds__(256)
+k_stencil_generic(Params P, const real_t *__restrict__ uc, const real_t *up,
+                  real_t *un)
+{
+    const long long li = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+#if SW_DIM == 3
+    const long long Iy = P.ny - 2 * R, Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    if (li >= Iy * Ix * Iz)
+        return;
+    const int k = R + (int)(li % Iy);
+    const int j = R + (int)((li / Iy) % Ix);
+    const int i = R + (int)(li / (Iy * Ix));
+    const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
+#else
+    const long long Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    if (li >= Ix * Iz)
+        return;
+    const int j = R + (int)(li % Ix);
+    const int i = R + (int)(li / Ix);
+    const long long p = (long long)i * P.sz + j;
+#endif
+    const real_t *c = uc + p;
+    real_t acc_c = P.c2[0] * c[0];
+#if SW_DIM == 3
+    real_t acc_m = P.c2[0] * c[0];
+#endif
+    real_t acc_z = P.c2[0] * c[0];
+#if SW_VD
+    const real_t *d = P.rho + p;
+    real_t du_c = 0, dr_c = 0, du_z = 0, dr_z = 0;
+#if SW_DIM == 3
+    real_t du_m = 0, dr_m = 0;
+#endif
+#endif
+#pragma unroll
+    for (int m = 1; m <= R; m++) {
+        acc_c += P.c2[m] * (c[m] + c[-m]);
+#if SW_VD
+        du_c += P.c1[m] * (c[m] - c[-m]);
+        dr_c += P.c1[m] * (d[m] - d[-m]);
+#endif
+#if SW_DIM == 3
+        {
+            const long long o = (long long)m * P.sx;
+            acc_m += P.c2[m] * (c[o] + c[-o]);
+#if SW_VD
+            const long long oq = (long long)m * P.nx;   // reference quirk
+            du_m += P.c1[m] * (c[oq] - c[-oq]);
+            dr_m += P.c1[m] * (d[oq] - d[-oq]);
+#endif
+        }
+#endif
+        {
+            const long long o = (long long)m * P.sz;
+            acc_z += P.c2[m] * (c[o] + c[-o]);
+#if SW_VD
+            du_z += P.c1[m] * (c[o] - c[-o]);
+            dr_z += P.c1[m] * (d[o] - d[-o]);
+#endif
+        }
+    }
+    real_t value = 0;
+#if SW_DIM == 3
+    value += acc_c / P.dy2 + acc_m / P.dx2 + acc_z / P.dz2;
+#if SW_VD
+    {
+        const real_t t_c = (du_c * dr_c) / (4 * P.dy2);
+        const real_t t_m = (du_m * dr_m) / (4 * P.dx2);
+        const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
+        value -= (t_c + t_m + t_z) / d[0];
+    }
+#endif
+#else
+    value += acc_c / P.dx2 + acc_z / P.dz2;
+#if SW_VD
+    {
+        const real_t t_c = (du_c * dr_c) / (4 * P.dx2);
+        const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
+        value -= (t_c + t_z) / d[0];
+    }
+#endif
+#endif
+    un[p] = leapfrog(value, P.q[p], P.e[p], c[0], up[p]);
+}
+
+// Source injection in the reference's sequential order: one workgroup walks
+// the sources in index order; the points of one box are distinct, so its
+// threads update them in parallel, and a barrier separates sources.
+// halo_only: skip interior points (the fused kernel adds those itself).
+// Reference: cd-3d wave.c:208-295.
+__global__ void __launch_bounds__(256)
+k_inject(Params P, real_t *un, int n, int halo_only)
+{
+    const int r = P.r;
+    for (int s = 0; s < P.nsrc; s++) {
+        const real_t amp = wavelet_amp(P, n, s);
+        if (amp != (real_t)0) {
+            const Box b = P.sbox[s];
+            const int cz = b.hi[0] - b.lo[0] + 1;
+            const int cx = b.hi[1] - b.lo[1] + 1;
+#if SW_DIM == 3
+            const int cy = b.hi[2] - b.lo[2] + 1;
+#else
+            const int cy = 1;
+#endif
+            const int cnt = cz * cx * cy;
+            for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                const int kk = t % cy;
+                const int jj = (t / cy) % cx;
+                const int ii = t / (cy * cx);
+                const int i = b.lo[0] + ii, j = b.lo[1] + jj;
+#if SW_DIM == 3
+                const int k = b.lo[2] + kk;
+                const bool interior = i >= r && i < P.nz - r && j >= r &&
+                                      j < P.nx - r && k >= r && k < P.ny - r;
+                const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
+#else
+                const int k = 0;
+                const bool interior = i >= r && i < P.nz - r && j >= r && j < P.nx - r;
+                const long long p = (long long)i * P.sz + j;
+#endif
+                if (halo_only && interior)
+                    continue;
+                const real_t kws = box_weight(b, P.sw, i, j, k);
+                un[p] += source_term(P.q[p], P.e[p], kws, amp);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One boundary phase: each thread owns one face line and applies the four
+// reference operations in order.  axis: 0 z, 1 middle x (3D), 2 contiguous.
+// Reference: cd-3d wave.c:324-480 (face_line semantics).
+__global__ void __launch_bounds__(256)
+k_bc_phase(Params P, real_t *un, int axis)
+{
+    const int r = P.r;
+    const long long li = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long base, st;
+    int n, before, after;
+#if SW_DIM == 3
+    const long long Iy = P.ny - 2 * r, Ix = P.nx - 2 * r, Iz = P.nz - 2 * r;
+    if (axis == 2) {          // lines along y over interior (i, j)
+        if (li >= Iz * Ix) return;
+        const int j = r + (int)(li % Ix), i = r + (int)(li / Ix);
+        base = (long long)i * P.sz + (long long)j * P.sx; st = 1; n = P.ny;
+        before = P.bc[4]; after = P.bc[5];
+    } else if (axis == 1) {   // lines along x over interior (i, k)
+        if (li >= Iz * Iy) return;
+        const int k = r + (int)(li % Iy), i = r + (int)(li / Iy);
+        base = (long long)i * P.sz + k; st = P.sx; n = P.nx;
+        before = P.bc[2]; after = P.bc[3];
+    } else {                  // lines along z over interior (j, k)
+        if (li >= Ix * Iy) return;
+        const int k = r + (int)(li % Iy), j = r + (int)(li / Iy);
+        base = (long long)j * P.sx + k; st = P.sz; n = P.nz;
+        before = P.bc[0]; after = P.bc[1];
+    }
+#else
+    const long long Ix = P.nx - 2 * r, Iz = P.nz - 2 * r;
+    if (axis == 2) {          // lines along x over interior i
+        if (li >= Iz) return;
+        base = (long long)(r + li) * P.sz; st = 1; n = P.nx;
+        before = P.bc[2]; after = P.bc[3];
+    } else {                  // lines along z over interior j
+        if (li >= Ix) return;
+        base = r + li; st = P.sz; n = P.nz;
+        before = P.bc[0]; after = P.bc[1];
+    }
+#endif
+    const long long lo = base + (long long)r * st;
+    const long long hi = base + (long long)(n - r - 1) * st;
+    if (before == 1)
+        un[lo] = 0;
+    if (before == 2)
+        for (int m = 1; m <= r; m++)
+            un[lo - m * st] = un[lo + m * st];
+    if (after == 1)
+        un[hi] = 0;
+    if (after == 2)
+        for (int m = 1; m <= r; m++)
+            un[hi + m * st] = un[hi - m * st];
+}
+
+// Receivers: one thread per receiver, sequential sum over its box in the
+// reference order (z, then x, then y).  Reads level n (the step's input).
+// Reference: cd-3d wave.c:499-566.
+__global__ void __launch_bounds__(256)
+k_receivers(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbox,
+            const real_t *__restrict__ rw, int nrec, real_t *__restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nrec)
+        return;
+    const Box b = rbox[q];
+    const int cz = b.hi[0] - b.lo[0] + 1;
+    const real_t *wz = rw + b.woff;
+    const real_t *wx = wz + cz;
+#if SW_DIM == 3
+    const int cx = b.hi[1] - b.lo[1] + 1;
+    const real_t *wy = wx + cx;
+#endif
+    real_t acc = 0;
+    for (int i = b.lo[0]; i <= b.hi[0]; i++) {
+        const real_t fz = wz[i - b.lo[0]];
+        for (int j = b.lo[1]; j <= b.hi[1]; j++) {
+            const real_t fzx = fz * wx[j - b.lo[1]];
+#if SW_DIM == 3
+            const real_t *row = uc + (long long)i * P.sz + (long long)j * P.sx;
+            for (int k = b.lo[2]; k <= b.hi[2]; k++)
+                acc += row[k] * (fzx * wy[k - b.lo[2]]);
+#else
+            acc += uc[(long long)i * P.sz + j] * fzx;
+#endif
+        }
+    }
+    out[q] = acc;
+}
+
+// Exchange two slots (the reference's even-stride swap, wave.c:578-611).
+__global__ void k_swap(real_t *a, real_t *b, long long npts)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         p < npts; p += (long long)gridDim.x * blockDim.x) {
+        const real_t t = a[p];
+        a[p] = b[p];
+        b[p] = t;
+    }
+}
+
+}  // namespace sw

@@ -1,0 +1,118 @@
+"""Device-resident forward runs.
+
+``DeviceForward`` keeps one experiment's wavefield slots, models and shot
+record in HBM and calls the library's device entry
+(``simwave_hip_forward_device_<variant>``, declared in include/simwave_hip.h)
+on a HIP stream, so repeated shots pay no host<->device traffic.  Device
+memory comes from PyTorch's caching allocator (plumbing only; every
+computation is in the HIP library).
+"""
+import ctypes
+
+import numpy as np
+
+from .middleware import Middleware
+
+__all__ = ["DeviceForward", "FLAG_TIME_STEPS", "FLAG_GENERIC"]
+
+FLAG_TIME_STEPS = 1   # accumulate the step kernel's time with HIP events
+FLAG_GENERIC = 2      # use the generic kernels instead of the fused one
+
+# arguments that live in device memory for the device entry
+_DEVICE_KEYS = ("u_full", "velocity_model", "density_model", "damping_mask",
+                "shot_record")
+
+
+def _variant(kwargs):
+    vel = kwargs["velocity_model"]
+    dens = "vd" if kwargs.get("density_model") is not None else "cd"
+    prec = {"float32": "f32", "float64": "f64"}[str(vel.dtype)]
+    return "%dd_%s_%s" % (vel.ndim, dens, prec)
+
+
+class DeviceForward:
+    """One experiment resident on a GPU.
+
+    Parameters
+    ----------
+    solver : simwave_amd.Solver
+        Supplies the frontend arrays (same inputs as Solver.forward).
+    device : str or torch.device, optional
+    """
+
+    def __init__(self, solver, device="cuda"):
+        import torch
+
+        self._torch = torch
+        self.solver = solver
+        kwargs = Middleware.normalize(**solver.forward_arguments())
+        self.variant = _variant(kwargs)
+        lib = solver._middleware.library(
+            dimension=kwargs["velocity_model"].ndim,
+            density=kwargs.get("density_model"),
+            dtype=kwargs["velocity_model"].dtype)
+        self._lib = lib
+        self._fn = getattr(lib, "simwave_hip_forward_device_" + self.variant)
+        self.device = torch.device(device)
+        self.tensors = {}
+        for key in _DEVICE_KEYS:
+            host = kwargs.get(key)
+            if host is None:
+                continue
+            self.tensors[key] = torch.from_numpy(np.ascontiguousarray(host)).to(
+                self.device)
+        self.kwargs = kwargs
+        argtypes, args = Middleware.ordered(kwargs)
+        keys = [k for k in Middleware._keys_in_order_static() if kwargs.get(k) is not None]
+        for pos, key in enumerate(keys):
+            if key in self.tensors:
+                argtypes[pos] = ctypes.c_void_p
+                args[pos] = ctypes.c_void_p(self.tensors[key].data_ptr())
+        self._fn.restype = ctypes.c_double
+        self._fn.argtypes = argtypes + [ctypes.c_void_p, ctypes.c_uint]
+        self._args = args
+
+    @property
+    def updates_per_shot(self):
+        """Interior grid-point updates of one shot (points x time steps)."""
+        kw = self.kwargs
+        r = kw["space_order"] // 2
+        pts = 1
+        for n in kw["velocity_model"].shape:
+            pts *= n - 2 * r
+        return pts * (kw["end_timestep"] - kw["begin_timestep"] + 1)
+
+    def reset(self):
+        """Zeroes the wavefield slots (a fresh shot)."""
+        self.tensors["u_full"].zero_()
+
+    def run(self, stream=None, flags=0):
+        """Runs one shot on `stream` (default: torch's current stream)."""
+        torch = self._torch
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        secs = self._fn(*self._args, ctypes.c_void_p(stream.cuda_stream),
+                        ctypes.c_uint(flags))
+        if secs < 0:
+            from .middleware import _last_error
+            raise RuntimeError("device forward failed: {}".format(
+                _last_error(self._lib)))
+        return secs
+
+    def timings(self):
+        """Phase timings of the last call on this thread:
+        [total s, prepare s, loop s, download s, step-kernel s, step launches,
+         fused(1)/generic(0), 0]."""
+        out = (ctypes.c_double * 8)()
+        fn = self._lib.simwave_hip_last_timings
+        fn.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        fn.restype = ctypes.c_int
+        fn(out, 8)
+        return list(out)
+
+    def results(self):
+        """(u_full, shot_record) as the Solver returns them."""
+        sm, tm = self.solver.space_model, self.solver.time_model
+        u = self.tensors["u_full"].cpu().numpy()
+        u = sm.remove_halo_region(tm.remove_time_halo_region(u))
+        return u, self.tensors["shot_record"].cpu().numpy()
