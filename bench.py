@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--time-steps", type=int, default=None,
+                    help="truncate each shot to this many time steps (profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -189,7 +191,7 @@ def main():
     t0 = time.time()
     parts = builder(dtype=dtype)
     solver = sw.Solver(**parts)
-    dev = DeviceForward(solver)
+    dev = DeviceForward(solver, end_timestep=args.time_steps)
     t_host = time.time() - t0
     kw = dev.kwargs
     steps_per_shot = kw["end_timestep"] - kw["begin_timestep"] + 1

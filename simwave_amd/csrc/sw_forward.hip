@@ -15,9 +15,12 @@
 // slots of HBM, and the even-stride content exchange becomes a swap of the two
 // device pointers.
 #include "sw_kernels.hpp"
+#include "sw_step3d.hpp"
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -82,6 +85,10 @@ double now_s()
 constexpr unsigned kFlagTimeSteps = 1u;    // accumulate per-launch step time
 constexpr unsigned kFlagGeneric = 2u;      // force the generic kernels
 constexpr int kMaxFusedSources = 64;
+// cooperative receiver kernel: receivers per block and points per round
+// (LDS = kRecChunk * kRecPerBlock values)
+constexpr int kRecPerBlock = sizeof(real_t) == 4 ? 16 : 8;
+constexpr int kRecChunk = 768;
 
 // All arguments of one forward call, normalised across variants.
 struct Args {
@@ -245,7 +252,23 @@ struct Prepared {
     bool halo_sources = false;   // some source box reaches the halo
     int r = 0;
     long long npts = 0;
+#if SW_DIM == 3
+    bool tiled = false;          // use k_step3d_tiled
+    void (*tiled_launch)(const struct Prepared &, const real_t *, const real_t *,
+                         real_t *, int, hipStream_t) = nullptr;
+    const Params *Pd = nullptr;   // device copy of P for the tiled kernel
+    const real_t *cC = nullptr, *cB = nullptr;   // update coefficients
+    const double *cA = nullptr;
+    bool fastdiv = false;
+    sw::FixedDiv dvz{}, dvx{}, dvy{};
+    sw::Tiling tiling{};
+    int tiled_blocks = 0;
+#endif
 };
+
+#if SW_DIM == 3
+void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st);
+#endif
 
 bool env_flag(const char *name)
 {
@@ -405,10 +428,243 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
         fused = false;
 #endif
     pr.fused = fused;
+#if SW_DIM == 3 && !SW_VD
+    if (fused && !env_flag("SIMWAVE_HIP_UNTILED"))
+        setup_tiling(pr, arena, st);
+#endif
     return pr;
 }
 
 // ------------------------------------------------------------ launches
+
+// Divisor data for sw::fixed_div: y = RN(1/b), chosen among the neighbours of
+// an extended-precision estimate by the smallest exact residual |1 - b*y|,
+// and the numerator magnitudes [lo, hi] for which q0 = RN(a*y) is normal,
+// r = a - b*q0 is exact and nothing overflows.  false: use IEEE division.
+bool make_fixed_div(real_t b, sw::FixedDiv &d)
+{
+    if (!(b > (real_t)0) || !std::isfinite(b))
+        return false;
+    int eb = 0;
+    (void)std::frexp(b, &eb);
+    eb -= 1;                                   // b in [2^eb, 2^(eb+1))
+    if (eb < -60 || eb > 60)
+        return false;
+    constexpr int p = std::numeric_limits<real_t>::digits;
+    constexpr int emin = std::numeric_limits<real_t>::min_exponent - 1;
+    constexpr int emax = std::numeric_limits<real_t>::max_exponent - 1;
+    const real_t y0 = (real_t)(1.0L / (long double)b);
+    const real_t cand[3] = {std::nextafter(y0, (real_t)0), y0,
+                            std::nextafter(y0, std::numeric_limits<real_t>::infinity())};
+    real_t best = y0;
+    double best_res = 1e300;
+    for (real_t y : cand) {
+        // float: b*y is exact in double; double: fma rounds the tiny residual once
+        const double res = std::fabs(std::fma(-(double)b, (double)y, 1.0));
+        if (res < best_res) {
+            best_res = res;
+            best = y;
+        }
+    }
+    d.b = b;
+    d.y = best;
+    d.lo = (real_t)std::ldexp(1.0, std::max(emin + eb + 2, emin + p));
+    d.hi = (real_t)std::ldexp(1.0, emax - 2 + std::min(0, eb));
+    return true;
+}
+
+#if SW_DIM == 3
+// Tile shape of the tiled kernel: the default per stencil radius, or
+// SIMWAVE_HIP_TILE (see tile_rpt / tile_nw).
+constexpr int default_tile(int) { return 0; }
+
+template <int R, int RPT, int NW>
+void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un,
+                  int n, hipStream_t st)
+{
+    const Params &P = pr.P;
+    sw::TiledArgs A{};
+    A.uc = uc;
+    A.up = up;
+    A.C = pr.cC;
+    A.B = pr.cB;
+    A.A = pr.cA;
+    A.un = un;
+    A.fastdiv = pr.fastdiv ? 1 : 0;
+    A.dvz = pr.dvz;
+    A.dvx = pr.dvx;
+    A.dvy = pr.dvy;
+    A.div_lo = std::max({pr.dvz.lo, pr.dvx.lo, pr.dvy.lo});
+    A.div_hi = std::min({pr.dvz.hi, pr.dvx.hi, pr.dvy.hi});
+    A.nz = P.nz;
+    A.nx = P.nx;
+    A.ny = P.ny;
+    A.pbytes = (unsigned)((long long)P.nx * P.ny * (long long)sizeof(real_t));
+    A.bc = 0;
+    for (int f = 0; f < 6; f++)
+        A.bc |= (P.bc[f] & 3) << (2 * f);
+    A.dz2 = P.dz2;
+    A.dx2 = P.dx2;
+    A.dy2 = P.dy2;
+    for (int m = 0; m <= sw::kMaxR; m++)
+        A.c2[m] = P.c2[m];
+    A.nsrc = P.nsrc;
+    A.sbox = P.sbox;
+    A.T = pr.tiling;
+    hipLaunchKernelGGL((sw::k_step3d_tiled<R, RPT, NW>), dim3(pr.tiled_blocks),
+                       dim3(64, NW), 0, st, A, pr.Pd, n);
+}
+
+template <int R, int RPT, int NW>
+int tiled_occupancy()
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void *)sw::k_step3d_tiled<R, RPT, NW>, 64 * NW, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu;
+}
+
+template <int TXR>
+void tiled_flags(const real_t *e, const Params &P, const sw::Tiling &T,
+                 unsigned *mask, hipStream_t st)
+{
+    hipLaunchKernelGGL((sw::k_damp_flags<TXR>), dim3(T.ncols, P.nz), dim3(256), 0,
+                       st, e, P.nz, P.nx, P.ny, P.r, T.gk, T.nwords, mask);
+}
+
+using TiledFn = void (*)(const Prepared &, const real_t *, const real_t *, real_t *, int,
+                         hipStream_t);
+using OccFn = int (*)();
+using FlagFn = void (*)(const real_t *, const Params &, const sw::Tiling &,
+                        unsigned *, hipStream_t);
+
+struct TiledKernel {
+    TiledFn launch;
+    OccFn occupancy;
+};
+
+// (rows per thread, waves per block) variants: 0 (2,4), 1 (1,8), 2 (2,8), 3 (4,4)
+constexpr int kTileShapes = 4;
+constexpr int tile_rpt(int v) { return v == 1 ? 1 : v == 3 ? 4 : 2; }
+constexpr int tile_nw(int v) { return v == 1 || v == 2 ? 8 : 4; }
+
+template <int R>
+constexpr TiledKernel tiled_entry(int v)
+{
+    return v == 1 ? TiledKernel{&tiled_launch<R, 1, 8>, &tiled_occupancy<R, 1, 8>}
+         : v == 2 ? TiledKernel{&tiled_launch<R, 2, 8>, &tiled_occupancy<R, 2, 8>}
+         : v == 3 ? TiledKernel{&tiled_launch<R, 4, 4>, &tiled_occupancy<R, 4, 4>}
+                  : TiledKernel{&tiled_launch<R, 2, 4>, &tiled_occupancy<R, 2, 4>};
+}
+
+TiledKernel tiled_kernel(int R, int rpt)
+{
+    switch (R) {
+    case 1: return tiled_entry<1>(rpt);
+    case 2: return tiled_entry<2>(rpt);
+    case 3: return tiled_entry<3>(rpt);
+    case 4: return tiled_entry<4>(rpt);
+    case 5: return tiled_entry<5>(rpt);
+    case 6: return tiled_entry<6>(rpt);
+    case 7: return tiled_entry<7>(rpt);
+    case 8: return tiled_entry<8>(rpt);
+    case 9: return tiled_entry<9>(rpt);
+    default: return tiled_entry<10>(rpt);
+    }
+}
+
+FlagFn tiled_flag_fn(int txr)
+{
+    return txr == 8 ? &tiled_flags<8> : txr == 16 ? &tiled_flags<16> : &tiled_flags<4>;
+}
+
+// Chooses the tile grid and the persistent block count, and marks the tile
+// planes that hold damping.
+void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
+{
+    const Params &P = pr.P;
+    const int R = pr.r;
+    int shape = env_int("SIMWAVE_HIP_TILE", default_tile(R));
+    shape = shape >= 0 && shape < kTileShapes ? shape : 0;
+    const TiledKernel K = tiled_kernel(R, shape);
+    const int TXR = tile_rpt(shape) * tile_nw(shape);
+    const int Iy = P.ny - 2 * R, Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    const long long pbytes = (long long)P.nx * P.ny * (long long)sizeof(real_t);
+    // buffer offsets are 32-bit: the planes one tile touches (at most
+    // 64 + 2R) must span less than 4 GiB
+    if (pbytes * (64 + 2 * R) >= (1ll << 32))
+        return;
+    sw::Tiling T{};
+    T.gk = (Iy + 63) / 64;
+    T.gj = (Ix + TXR - 1) / TXR;
+    T.ncols = T.gk * T.gj;
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+              "hipDeviceGetAttribute");
+    const int per_cu = std::max(1, env_int("SIMWAVE_HIP_BLOCKS_PER_CU",
+                                           K.occupancy()));
+    const long long resident = (long long)per_cu * cus;
+    int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
+    if (zlen <= 0) {
+        // fewest rounds x (zlen + prologue planes) over zlen <= 64
+        double best = 1e300;
+        for (int gz = (Iz + 63) / 64; gz <= std::max(1, Iz / 8); gz++) {
+            const int zl = (Iz + gz - 1) / gz;
+            const long long nt = (long long)T.ncols * ((Iz + zl - 1) / zl);
+            const long long blocks = std::min<long long>(nt, resident);
+            const double rounds = std::ceil((double)nt / (double)blocks);
+            const double cost = rounds * (zl + 2 * R + 4);
+            if (cost < best - 1e-9) {
+                best = cost;
+                zlen = zl;
+            }
+        }
+    }
+    zlen = std::max(1, std::min({zlen, Iz, 64}));   // one 64-bit damping mask per tile
+    T.zlen = zlen;
+    T.gz = (Iz + zlen - 1) / zlen;
+    T.ntiles = T.ncols * T.gz;
+    const long long blocks =
+        std::max<long long>(8, std::min<long long>(T.ntiles, resident) / 8 * 8);
+    T.nwords = P.nz / 32 + 3;
+    unsigned *mask = arena.alloc<unsigned>((size_t)T.ncols * T.nwords);
+    {
+        const long long words = (long long)T.ncols * T.nwords;
+        hipLaunchKernelGGL(sw::k_zero_u32, dim3((unsigned)std::min<long long>((words + 255) / 256, 4096)),
+                           dim3(256), 0, st, mask, words);
+    }
+    T.dmask = mask;
+    tiled_flag_fn(TXR)(P.e, P, T, mask, st);
+    hip_check(hipGetLastError(), "k_damp_flags launch");
+    Params *Pd = arena.alloc<Params>(1);
+    hip_check(hipMemcpyAsync(Pd, &P, sizeof(Params), hipMemcpyHostToDevice, st),
+              "copy params");
+    pr.Pd = Pd;
+    {
+        const long long npts = pr.npts;
+        real_t *C = arena.alloc<real_t>(npts);
+        real_t *B = arena.alloc<real_t>(npts);
+        double *Aa = arena.alloc<double>(npts);
+        const int nb = (int)std::min<long long>((npts + 255) / 256, 8192);
+        hipLaunchKernelGGL(sw::k_update_coeffs, dim3(nb), dim3(256), 0, st, P.q, P.e, C, B,
+                           Aa, npts);
+        hip_check(hipGetLastError(), "k_update_coeffs launch");
+        pr.cC = C;
+        pr.cB = B;
+        pr.cA = Aa;
+    }
+    pr.fastdiv = !env_flag("SIMWAVE_HIP_IEEE_DIV") && make_fixed_div(P.dz2, pr.dvz) &&
+                 make_fixed_div(P.dx2, pr.dvx) && make_fixed_div(P.dy2, pr.dvy);
+    pr.tiled_launch = K.launch;
+    pr.tiling = T;
+    pr.tiled_blocks = (int)blocks;
+    pr.tiled = true;
+}
+#endif
 
 #if SW_DIM == 3
 constexpr int kTX = 8;
@@ -526,6 +782,8 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
         hip_check(hipEventCreate(&ev1), "hipEventCreate");
     }
     const unsigned rec_blocks = (unsigned)((a.nrec + 255) / 256);
+    const bool coop_receivers = !env_flag("SIMWAVE_HIP_REC_THREAD");
+    const unsigned coop_blocks = (unsigned)((a.nrec + kRecPerBlock - 1) / kRecPerBlock);
     for (size_t idx = 0; idx < plan.size(); idx++) {
         const size_t n = a.n0 + idx;
         const StepSlots &s = plan[idx];
@@ -539,7 +797,12 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
                                    (int)n, 1);
             if (timed)
                 hip_check(hipEventRecord(ev0, st), "hipEventRecord");
-            fused(P, uc, up, un, (int)n, st);
+#if SW_DIM == 3
+            if (pr.tiled)
+                pr.tiled_launch(pr, uc, up, un, (int)n, st);
+            else
+#endif
+                fused(P, uc, up, un, (int)n, st);
             if (timed)
                 hip_check(hipEventRecord(ev1, st), "hipEventRecord");
         } else {
@@ -553,10 +816,16 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
                                    (int)n, 0);
             launch_bc_phases(P, un, st);
         }
-        if (a.nrec)
-            hipLaunchKernelGGL(sw::k_receivers, dim3(rec_blocks), dim3(256), 0, st, P,
-                               uc, pr.rbox, pr.rw, (int)a.nrec,
-                               rec_dev + (n - 1 - rec_row0) * a.nrec);
+        if (a.nrec) {
+            real_t *row = rec_dev + (n - 1 - rec_row0) * a.nrec;
+            if (coop_receivers)
+                hipLaunchKernelGGL((sw::k_receivers_coop<kRecPerBlock, kRecChunk>),
+                                   dim3(coop_blocks), dim3(256), 0, st, P, uc, pr.rbox,
+                                   pr.rw, (int)a.nrec, row);
+            else
+                hipLaunchKernelGGL(sw::k_receivers, dim3(rec_blocks), dim3(256), 0, st, P,
+                                   uc, pr.rbox, pr.rw, (int)a.nrec, row);
+        }
         hip_check(hipGetLastError(), "step launch");
         if (timed) {
             float ms = 0.f;
@@ -866,6 +1135,20 @@ SW_EXPORT int simwave_hip_last_timings(double *out, int n)
     for (int i = 0; i < m; i++)
         out[i] = g_timings[i];
     return m;
+}
+
+// Host-only helper (no device work): the fixed-divisor parameters the
+// kernels use for a divisor b.  Returns 0 when IEEE division is used instead.
+SW_EXPORT int SW_SUFFIX(simwave_hip_fixed_div_)(real_t b, real_t *y, real_t *lo,
+                                                 real_t *hi)
+{
+    sw::FixedDiv d{};
+    if (!make_fixed_div(b, d))
+        return 0;
+    *y = d.y;
+    *lo = d.lo;
+    *hi = d.hi;
+    return 1;
 }
 
 SW_EXPORT const char *simwave_hip_variant(void)

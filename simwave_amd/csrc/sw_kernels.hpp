@@ -91,6 +91,46 @@ __device__ __forceinline__ real_t leapfrog(real_t value, real_t q, real_t e,
     return (real_t)t;
 }
 
+// Leap-frog update from per-point coefficients precomputed once per run:
+// C = (dt2/s)/den, B = num/den (working type) and A = 2.0/den (double), the
+// values the reference recomputes at every step (wave.c:177-185).  With
+// e == 0 they are C == q, B == 1, A == 2 exactly.
+__device__ __forceinline__ real_t leapfrog_coef(real_t value, real_t C, real_t B,
+                                                double A, real_t uc, real_t up)
+{
+    value = value * C;
+    double t = A * (double)uc - (double)(B * up);
+    t = t + (double)value;
+    return (real_t)t;
+}
+
+// Correctly rounded a / b for a fixed divisor b (Markstein's theorem):
+// with y = RN(1/b), q0 = RN(a*y), r = a - b*q0 is exact and RN(q0 + r*y) is
+// RN(a/b), provided nothing under- or overflows.  `lo`/`hi` bound |a| so that
+// this holds (set on the host per divisor); zero is also safe (a -0 result
+// may come out +0, which no later operation distinguishes).
+struct FixedDiv {
+    real_t b, y, lo, hi;
+};
+
+__device__ __forceinline__ bool fixed_div_ok(real_t a, const FixedDiv &d)
+{
+    const real_t m = a < (real_t)0 ? -a : a;
+    return (m >= d.lo && m <= d.hi) || a == (real_t)0;
+}
+
+__device__ __forceinline__ real_t fixed_div(real_t a, const FixedDiv &d)
+{
+    const real_t q0 = a * d.y;
+#if defined(SW_DOUBLE)
+    const real_t r = __builtin_fma(-q0, d.b, a);
+    return __builtin_fma(r, d.y, q0);
+#else
+    const real_t r = __builtin_fmaf(-q0, d.b, a);
+    return __builtin_fmaf(r, d.y, q0);
+#endif
+}
+
 // Source contribution at one point: dt2/s * kws * w / den.
 // Reference: cd-3d wave.c:272-277.
 __device__ __forceinline__ real_t source_term(real_t q, real_t e, real_t kws,
@@ -173,19 +213,32 @@ __global__ void k_prepare(const real_t *vel, const real_t *damp,
     }
 }
 
+// The per-point update coefficients C, B, A from q and e (see leapfrog_coef).
+__global__ void k_update_coeffs(const real_t *__restrict__ q, const real_t *__restrict__ e,
+                                real_t *__restrict__ C, real_t *__restrict__ B,
+                                double *__restrict__ A, long long npts)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         p < npts; p += (long long)gridDim.x * blockDim.x) {
+        const real_t ev = e[p];
+        const real_t den = (real_t)(1.0 + (double)ev);
+        const real_t num = (real_t)(1.0 - (double)ev);
+        C[p] = q[p] / den;
+        B[p] = num / den;
+        A[p] = 2.0 / (double)den;
+    }
+}
+
 // -------------------------------------------------- boundary predicates
-// Value a point carries after the boundary phases that precede a given read,
-// for grids whose interior spans at least r+1 points on every axis (then the
-// Neumann mirrors only read interior points and only write halo points).
+// For grids whose interior spans at least r+1 points on every axis, the
+// Neumann mirrors only read interior points and only write halo points, so
+// the three boundary phases of one step can be applied pointwise by the
+// thread that owns the mirrored interior point.  A value read by a later
+// phase is zero iff an earlier phase's Dirichlet face zeroed it.
 // Reference order: contiguous axis, middle axis (3D), z; each face line
 // applies before-Dirichlet, before-Neumann, after-Dirichlet, after-Neumann.
 // Reference: cd-3d wave.c:304-480, cd-2d wave.c:280-393.
-struct BcFlags {
-    bool zc;  // zeroed by a Dirichlet face of the contiguous axis
-    bool zm;  // ... of the middle axis (3D)
-    bool zz;  // ... of the z axis
-};
-
+//
 // Writes the final value of interior point (i,j,k) and every halo point the
 // Neumann faces copy from it.  v is the post-source value.
 __device__ __forceinline__ void store_with_bc(const Params &P, real_t *un,
@@ -728,6 +781,109 @@ k_receivers(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbo
         }
     }
     out[q] = acc;
+}
+
+// Receivers, cooperative form: a block serves RPB receivers.  All 256
+// threads gather box values and multiply them by their weights into LDS (CH
+// points per receiver per round), then lane q of the first wave adds its
+// receiver's products in box order.  Every product and every partial sum is
+// rounded exactly as in the reference's sequential loop (wave.c:533-565); only
+// the loads and multiplies run in parallel.
+template <int RPB, int CH>
+__global__ void __launch_bounds__(256)
+k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbox,
+                 const real_t *__restrict__ rw, int nrec, real_t *__restrict__ out)
+{
+    __shared__ real_t prod[CH][RPB];
+    __shared__ Box boxes[RPB];
+    __shared__ int counts[RPB];
+    __shared__ int maxcount;
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * RPB;
+    if (tid == 0)
+        maxcount = 0;
+    __syncthreads();
+    if (tid < RPB) {
+        const int q = r0 + tid;
+        Box b{};
+        int cnt = 0;
+        if (q < nrec) {
+            b = rbox[q];
+            cnt = (b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1);
+#if SW_DIM == 3
+            cnt *= b.hi[2] - b.lo[2] + 1;
+#endif
+        }
+        boxes[tid] = b;
+        counts[tid] = cnt;
+        atomicMax(&maxcount, cnt);
+    }
+    __syncthreads();
+    const int total_max = maxcount;
+    constexpr int STEP = 256 / RPB;       // points advanced per thread per pass
+    const int l = tid % RPB;              // receiver served by this thread
+    const Box b = boxes[l];
+    const int cnt = counts[l];
+    const int cz = b.hi[0] - b.lo[0] + 1, cx = b.hi[1] - b.lo[1] + 1;
+#if SW_DIM == 3
+    const int cy = b.hi[2] - b.lo[2] + 1;
+#else
+    const int cy = 1;
+#endif
+    const real_t *wz = rw + b.woff;
+    const real_t *wx = wz + cz;
+#if SW_DIM == 3
+    const real_t *wy = wx + cx;
+#endif
+    real_t acc = 0;
+    for (int base = 0; base < total_max; base += CH) {
+        // this thread's first point of the round, decoded once, then walked
+        int t = base + tid / RPB;
+        int ii = 0, jj = 0, kk = 0;
+        if (t < cnt) {
+            ii = t / (cx * cy);
+            const int rem = t - ii * cx * cy;
+            jj = rem / cy;
+            kk = rem - jj * cy;
+        }
+        for (int c = tid / RPB; c < CH && t < cnt; c += STEP, t += STEP) {
+#if SW_DIM == 3
+            const real_t kws = wz[ii] * wx[jj] * wy[kk];
+            const long long p = (long long)(b.lo[0] + ii) * P.sz +
+                                (long long)(b.lo[1] + jj) * P.sx + (b.lo[2] + kk);
+#else
+            const real_t kws = wz[ii] * wx[jj];
+            const long long p = (long long)(b.lo[0] + ii) * P.sz + (b.lo[1] + jj);
+#endif
+            prod[c][l] = uc[p] * kws;
+            kk += STEP;
+            while (kk >= cy) {
+                kk -= cy;
+                if (++jj >= cx) {
+                    jj = 0;
+                    ii++;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < RPB) {
+            const int nround = min(CH, cnt - base);
+            for (int c = 0; c < nround; c++)
+                acc += prod[c][tid];
+        }
+        __syncthreads();
+    }
+    if (tid < RPB && r0 + tid < nrec)
+        out[r0 + tid] = acc;
+}
+
+// Zero-fills a word array (a kernel rather than a memset, so it behaves the
+// same inside captured graphs).
+__global__ void k_zero_u32(unsigned *p, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 // Exchange two slots (the reference's even-stride swap, wave.c:578-611).

@@ -40,12 +40,14 @@ class DeviceForward:
     device : str or torch.device, optional
     """
 
-    def __init__(self, solver, device="cuda"):
+    def __init__(self, solver, device="cuda", end_timestep=None):
         import torch
 
         self._torch = torch
         self.solver = solver
         kwargs = Middleware.normalize(**solver.forward_arguments())
+        if end_timestep is not None:
+            kwargs["end_timestep"] = min(int(end_timestep), kwargs["end_timestep"])
         self.variant = _variant(kwargs)
         lib = solver._middleware.library(
             dimension=kwargs["velocity_model"].ndim,

@@ -102,3 +102,16 @@ def test_parity(gpu, name):
 def test_parity_generic_path(gpu, name):
     print("case", name, "(generic kernels)", flush=True)
     _compare(name, CASES[name], generic=True)
+
+
+@pytest.mark.parametrize("rpt", ["2", "3", "4"])
+@pytest.mark.parametrize("name", ["3d_cd_so2", "3d_cd_so8", "3d_cd_so20",
+                                  "3d_bc_mixed", "3d_src_halo", "3d_rect"])
+def test_parity_tiled_rows(gpu, name, rpt):
+    """Every rows-per-thread variant of the tiled 3D kernel."""
+    print("case", name, "rpt", rpt, flush=True)
+    os.environ["SIMWAVE_HIP_RPT"] = rpt
+    try:
+        _compare(name, CASES[name], generic=False)
+    finally:
+        os.environ.pop("SIMWAVE_HIP_RPT", None)
