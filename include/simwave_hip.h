@@ -1,0 +1,423 @@
+/*
+ * simwave_hip.h - C ABI of the HIP (gfx950) acoustic forward operator.
+ *
+ * One shared library per reference variant (dimension x density x
+ * precision): simwave_amd/lib/libsimwave_hip_<dim>d_<cd|vd>_<f32|f64>.so.
+ * Each library exports
+ *
+ *   - `forward`: the reference's FFI symbol with the reference's signature
+ *     for that variant (the one spelled out below as
+ *     simwave_hip_forward_<variant>), so the reference's ctypes loader
+ *     (simwave/kernel/backend/middleware.py:26-51,107-158) binds it
+ *     unchanged;
+ *   - simwave_hip_forward_<variant>: the same function under a unique name;
+ *   - simwave_hip_forward_device_<variant>: the same contract with device
+ *     pointers for the arrays that live on the GPU across calls;
+ *   - simwave_hip_fixed_div_<variant>, simwave_hip_last_error,
+ *     simwave_hip_last_timings, simwave_hip_variant.
+ *
+ * Array layout (reference contract, middleware.py:160-285): C-contiguous,
+ * last axis fastest; u is [num_snapshots][nz][nx][ny] (2D: no ny);
+ * receivers is [end_timestep - begin_timestep + 1 rows used][num_receivers],
+ * row n-1 written at step n.  Index arrays are size_t (numpy uint64).
+ *
+ * Errors: the reference has no error return (its C never checks).  These
+ * entry points return a negative value on failure (bad sizes, unsupported
+ * order, no device, HIP error) and leave a message for
+ * simwave_hip_last_error() on the calling thread; they never exit().
+ * Return value on success: wall seconds of the call, like the reference.
+ *
+ * Threading: synchronous; re-entrant on disjoint buffers.
+ */
+#ifndef SIMWAVE_HIP_H
+#define SIMWAVE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 2d_cd_f32 */
+/* Host arrays, as the reference's forward(): reference constant_density/2d/wave.c:23-36
+ * (`forward` in libsimwave_hip_2d_cd_f32.so has exactly this signature). */
+double simwave_hip_forward_2d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference constant_density/2d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_2d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_2d_cd_f32(float b, float *y, float *lo, float *hi);
+
+/* ---- 2d_cd_f64 */
+/* Host arrays, as the reference's forward(): reference constant_density/2d/wave.c:23-36
+ * (`forward` in libsimwave_hip_2d_cd_f64.so has exactly this signature). */
+double simwave_hip_forward_2d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference constant_density/2d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_2d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_2d_cd_f64(double b, double *y, double *lo, double *hi);
+
+/* ---- 2d_vd_f32 */
+/* Host arrays, as the reference's forward(): reference variable_density/2d/wave.c:23-36
+ * (`forward` in libsimwave_hip_2d_vd_f32.so has exactly this signature). */
+double simwave_hip_forward_2d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, density, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference variable_density/2d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_2d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_2d_vd_f32(float b, float *y, float *lo, float *hi);
+
+/* ---- 2d_vd_f64 */
+/* Host arrays, as the reference's forward(): reference variable_density/2d/wave.c:23-36
+ * (`forward` in libsimwave_hip_2d_vd_f64.so has exactly this signature). */
+double simwave_hip_forward_2d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, density, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference variable_density/2d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_2d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_2d_vd_f64(double b, double *y, double *lo, double *hi);
+
+/* ---- 3d_cd_f32 */
+/* Host arrays, as the reference's forward(): reference constant_density/3d/wave.c:23-36
+ * (`forward` in libsimwave_hip_3d_cd_f32.so has exactly this signature). */
+double simwave_hip_forward_3d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference constant_density/3d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_3d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_3d_cd_f32(float b, float *y, float *lo, float *hi);
+
+/* ---- 3d_cd_f64 */
+/* Host arrays, as the reference's forward(): reference constant_density/3d/wave.c:23-36
+ * (`forward` in libsimwave_hip_3d_cd_f64.so has exactly this signature). */
+double simwave_hip_forward_3d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference constant_density/3d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_3d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_3d_cd_f64(double b, double *y, double *lo, double *hi);
+
+/* ---- 3d_vd_f32 */
+/* Host arrays, as the reference's forward(): reference variable_density/3d/wave.c:23-36
+ * (`forward` in libsimwave_hip_3d_vd_f32.so has exactly this signature). */
+double simwave_hip_forward_3d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, density, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference variable_density/3d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_3d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_3d_vd_f32(float b, float *y, float *lo, float *hi);
+
+/* ---- 3d_vd_f64 */
+/* Host arrays, as the reference's forward(): reference variable_density/3d/wave.c:23-36
+ * (`forward` in libsimwave_hip_3d_vd_f64.so has exactly this signature). */
+double simwave_hip_forward_3d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots);
+/* Device arrays for u, velocity, density, damp and receivers (the
+ * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
+ * stream) and returns after the stream has drained.  The models are only
+ * read.  flags: 1 time the step kernel with HIP events, 2 force the
+ * generic kernels.  Same arguments as
+ * reference variable_density/3d/wave.c:23-36, plus stream and flags. */
+double simwave_hip_forward_device_3d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Host-only helper (no device work, no reference counterpart): the
+ * fixed-divisor parameters the kernels use for the divisor b (a squared
+ * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
+ * numerator range [lo, hi], or 0 when IEEE division is used. */
+int simwave_hip_fixed_div_3d_vd_f64(double b, double *y, double *lo, double *hi);
+
+/* ---- every library */
+/* Message of the last failed call on this thread ("" after a success). */
+const char *simwave_hip_last_error(void);
+/* Phase timings of the last call on this thread: total s, prepare s, time
+ * loop s, download s, step-kernel s (flags & 1), step launches, 1 when the
+ * fused kernels ran, 0.  Writes min(n, 8) values; returns that count. */
+int simwave_hip_last_timings(double *out, int n);
+/* The library's variant, e.g. "3d_cd_f32". */
+const char *simwave_hip_variant(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMWAVE_HIP_H */

@@ -253,12 +253,10 @@ struct Prepared {
     int r = 0;
     long long npts = 0;
 #if SW_DIM == 3
-    bool tiled = false;          // use k_step3d_tiled
+    bool tiled = false;          // use k_step3d_vec
     void (*tiled_launch)(const struct Prepared &, const real_t *, const real_t *,
                          real_t *, int, hipStream_t) = nullptr;
     const Params *Pd = nullptr;   // device copy of P for the tiled kernel
-    const real_t *cC = nullptr, *cB = nullptr;   // update coefficients
-    const double *cA = nullptr;
     bool fastdiv = false;
     sw::FixedDiv dvz{}, dvx{}, dvy{};
     sw::Tiling tiling{};
@@ -282,17 +280,14 @@ int env_int(const char *name, int dflt)
     return (v && *v) ? atoi(v) : dflt;
 }
 
-// Validates the call and uploads the run constants.  `dev_models` tells
-// whether vel/rho/damp are device pointers (device entry) or host pointers.
-Prepared prepare(const Args &a, bool dev_models, unsigned flags,
-                 DeviceArena &arena, hipStream_t st)
+// Argument checks that need no device; run before any HIP call so a bad
+// call reports its own error on any host.
+void validate(const Args &a)
 {
-    Prepared pr;
     const size_t so = a.space_order;
     if (so < 2 || so > 20 || so % 2)
         fail("space_order %zu: even orders 2..20 are supported", so);
     const int r = (int)(so / 2);
-    pr.r = r;
     const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : 1};
     for (int d = 0; d < SW_DIM; d++)
         if (dims[d] < (size_t)(2 * r) || dims[d] > (size_t)0x7fffffff)
@@ -317,6 +312,18 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
         fail("NULL receivers array");
     if (a.nsrc && !a.wavelet)
         fail("NULL wavelet array");
+}
+
+// Uploads the run constants (the arguments passed validate()).
+// `dev_models` tells whether vel/rho/damp are device pointers (device
+// entry) or host pointers.
+Prepared prepare(const Args &a, bool dev_models, unsigned flags,
+                 DeviceArena &arena, hipStream_t st)
+{
+    Prepared pr;
+    const int r = (int)(a.space_order / 2);
+    pr.r = r;
+    const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : 1};
 
     const long long npts = (long long)dims[0] * (long long)dims[1] * (long long)dims[2];
     pr.npts = npts;
@@ -474,21 +481,23 @@ bool make_fixed_div(real_t b, sw::FixedDiv &d)
 }
 
 #if SW_DIM == 3
-// Tile shape of the tiled kernel: the default per stencil radius, or
-// SIMWAVE_HIP_TILE (see tile_rpt / tile_nw).
-constexpr int default_tile(int) { return 0; }
+// Tile configurations of k_step3d_vec as (rows per wavefront WR, rows per
+// thread RPT, wavefronts per block NW).  Radius 4 (space order 8) carries
+// every configuration; other radii carry 0, 4 and 8.  SIMWAVE_HIP_TILE picks
+// one; otherwise default_tile(R).
+constexpr int kTileShapes = 12;
+constexpr int default_tile(int R) { return R == 4 ? 10 : R < 4 ? 0 : 4; }
 
-template <int R, int RPT, int NW>
+template <int R, int WR, int RPT, int NW>
 void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un,
                   int n, hipStream_t st)
 {
     const Params &P = pr.P;
-    sw::TiledArgs A{};
+    sw::StepArgs A{};
     A.uc = uc;
     A.up = up;
-    A.C = pr.cC;
-    A.B = pr.cB;
-    A.A = pr.cA;
+    A.q = P.q;
+    A.e = P.e;
     A.un = un;
     A.fastdiv = pr.fastdiv ? 1 : 0;
     A.dvz = pr.dvz;
@@ -511,152 +520,122 @@ void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t
     A.nsrc = P.nsrc;
     A.sbox = P.sbox;
     A.T = pr.tiling;
-    hipLaunchKernelGGL((sw::k_step3d_tiled<R, RPT, NW>), dim3(pr.tiled_blocks),
+    hipLaunchKernelGGL((sw::k_step3d_vec<R, WR, RPT, NW>), dim3(pr.tiled_blocks),
                        dim3(64, NW), 0, st, A, pr.Pd, n);
-}
-
-template <int R, int RPT, int NW>
-int tiled_occupancy()
-{
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, (const void *)sw::k_step3d_tiled<R, RPT, NW>, 64 * NW, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    return per_cu;
-}
-
-template <int TXR>
-void tiled_flags(const real_t *e, const Params &P, const sw::Tiling &T,
-                 unsigned *mask, hipStream_t st)
-{
-    hipLaunchKernelGGL((sw::k_damp_flags<TXR>), dim3(T.ncols, P.nz), dim3(256), 0,
-                       st, e, P.nz, P.nx, P.ny, P.r, T.gk, T.nwords, mask);
 }
 
 using TiledFn = void (*)(const Prepared &, const real_t *, const real_t *, real_t *, int,
                          hipStream_t);
-using OccFn = int (*)();
-using FlagFn = void (*)(const real_t *, const Params &, const sw::Tiling &,
-                        unsigned *, hipStream_t);
-
 struct TiledKernel {
     TiledFn launch;
-    OccFn occupancy;
+    int ty, txr;
 };
 
-// (rows per thread, waves per block) variants: 0 (2,4), 1 (1,8), 2 (2,8), 3 (4,4)
-constexpr int kTileShapes = 4;
-constexpr int tile_rpt(int v) { return v == 1 ? 1 : v == 3 ? 4 : 2; }
-constexpr int tile_nw(int v) { return v == 1 || v == 2 ? 8 : 4; }
-
-template <int R>
-constexpr TiledKernel tiled_entry(int v)
+template <int R, int WR, int RPT, int NW>
+TiledKernel tiled_config()
 {
-    return v == 1 ? TiledKernel{&tiled_launch<R, 1, 8>, &tiled_occupancy<R, 1, 8>}
-         : v == 2 ? TiledKernel{&tiled_launch<R, 2, 8>, &tiled_occupancy<R, 2, 8>}
-         : v == 3 ? TiledKernel{&tiled_launch<R, 4, 4>, &tiled_occupancy<R, 4, 4>}
-                  : TiledKernel{&tiled_launch<R, 2, 4>, &tiled_occupancy<R, 2, 4>};
+    using S = sw::TileShape<R, WR, RPT, NW>;
+    return TiledKernel{&tiled_launch<R, WR, RPT, NW>, S::TY, S::TXR};
 }
 
-TiledKernel tiled_kernel(int R, int rpt)
+template <int R>
+TiledKernel tiled_entry(int v)
 {
-    switch (R) {
-    case 1: return tiled_entry<1>(rpt);
-    case 2: return tiled_entry<2>(rpt);
-    case 3: return tiled_entry<3>(rpt);
-    case 4: return tiled_entry<4>(rpt);
-    case 5: return tiled_entry<5>(rpt);
-    case 6: return tiled_entry<6>(rpt);
-    case 7: return tiled_entry<7>(rpt);
-    case 8: return tiled_entry<8>(rpt);
-    case 9: return tiled_entry<9>(rpt);
-    default: return tiled_entry<10>(rpt);
+    if constexpr (R == 4) {
+        switch (v) {
+        case 1: return tiled_config<R, 1, 2, 4>();
+        case 2: return tiled_config<R, 1, 1, 8>();
+        case 3: return tiled_config<R, 2, 1, 4>();
+        case 4: return tiled_config<R, 4, 1, 4>();
+        case 5: return tiled_config<R, 2, 2, 4>();
+        case 6: return tiled_config<R, 1, 2, 8>();
+        case 7: return tiled_config<R, 4, 1, 2>();
+        case 8: return tiled_config<R, 8, 1, 1>();
+        case 9: return tiled_config<R, 8, 1, 2>();
+        case 10: return tiled_config<R, 4, 2, 1>();
+        case 11: return tiled_config<R, 4, 1, 1>();
+        default: return tiled_config<R, 1, 1, 4>();
+        }
+    } else {
+        return v == 4 ? tiled_config<R, 4, 1, 4>()
+             : v == 8 ? tiled_config<R, 8, 1, 1>()
+                      : tiled_config<R, 1, 1, 4>();
     }
 }
 
-FlagFn tiled_flag_fn(int txr)
+TiledKernel tiled_kernel(int R, int v)
 {
-    return txr == 8 ? &tiled_flags<8> : txr == 16 ? &tiled_flags<16> : &tiled_flags<4>;
+    switch (R) {
+    case 1: return tiled_entry<1>(v);
+    case 2: return tiled_entry<2>(v);
+    case 3: return tiled_entry<3>(v);
+    case 4: return tiled_entry<4>(v);
+    case 5: return tiled_entry<5>(v);
+    case 6: return tiled_entry<6>(v);
+    case 7: return tiled_entry<7>(v);
+    case 8: return tiled_entry<8>(v);
+    case 9: return tiled_entry<9>(v);
+    default: return tiled_entry<10>(v);
+    }
 }
 
-// Chooses the tile grid and the persistent block count, and marks the tile
-// planes that hold damping.
+// Chooses the tile grid and launch order and marks the tile planes that
+// hold damping.  Leaves pr.tiled false when the grid does not fit the
+// kernel's 32-bit buffer offsets.
 void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
 {
     const Params &P = pr.P;
     const int R = pr.r;
     int shape = env_int("SIMWAVE_HIP_TILE", default_tile(R));
-    shape = shape >= 0 && shape < kTileShapes ? shape : 0;
+    shape = shape >= 0 && shape < kTileShapes ? shape : default_tile(R);
     const TiledKernel K = tiled_kernel(R, shape);
-    const int TXR = tile_rpt(shape) * tile_nw(shape);
     const int Iy = P.ny - 2 * R, Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
     const long long pbytes = (long long)P.nx * P.ny * (long long)sizeof(real_t);
     // buffer offsets are 32-bit: the planes one tile touches (at most
-    // 64 + 2R) must span less than 4 GiB
-    if (pbytes * (64 + 2 * R) >= (1ll << 32))
+    // 64 + 2R) plus one row of overrun must stay below 2 GiB
+    if (pbytes * (64 + 2 * R + 1) >= (1ll << 31) || P.ny < sw::kVec)
         return;
     sw::Tiling T{};
-    T.gk = (Iy + 63) / 64;
-    T.gj = (Ix + TXR - 1) / TXR;
+    T.ty = K.ty;
+    T.txr = K.txr;
+    T.gk = (Iy + K.ty - 1) / K.ty;
+    T.gj = (Ix + K.txr - 1) / K.txr;
     T.ncols = T.gk * T.gj;
-    int dev = 0, cus = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-              "hipDeviceGetAttribute");
-    const int per_cu = std::max(1, env_int("SIMWAVE_HIP_BLOCKS_PER_CU",
-                                           K.occupancy()));
-    const long long resident = (long long)per_cu * cus;
+    T.order = env_int("SIMWAVE_HIP_ORDER", 1) == 1 ? 1 : 0;
+    T.chunk = (T.ncols + 7) / 8;
+    // z chunks of about 12 planes: short tiles keep many independent blocks
+    // in every phase (prologue, march) on each CU, which overlaps their
+    // loads and arithmetic better than long marches, at 2R/zlen extra
+    // centre-plane reads
     int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
     if (zlen <= 0) {
-        // fewest rounds x (zlen + prologue planes) over zlen <= 64
-        double best = 1e300;
-        for (int gz = (Iz + 63) / 64; gz <= std::max(1, Iz / 8); gz++) {
-            const int zl = (Iz + gz - 1) / gz;
-            const long long nt = (long long)T.ncols * ((Iz + zl - 1) / zl);
-            const long long blocks = std::min<long long>(nt, resident);
-            const double rounds = std::ceil((double)nt / (double)blocks);
-            const double cost = rounds * (zl + 2 * R + 4);
-            if (cost < best - 1e-9) {
-                best = cost;
-                zlen = zl;
-            }
-        }
+        const int chunks = std::max(1, (Iz + 11) / 12);
+        zlen = (Iz + chunks - 1) / chunks;
     }
     zlen = std::max(1, std::min({zlen, Iz, 64}));   // one 64-bit damping mask per tile
     T.zlen = zlen;
     T.gz = (Iz + zlen - 1) / zlen;
     T.ntiles = T.ncols * T.gz;
     const long long blocks =
-        std::max<long long>(8, std::min<long long>(T.ntiles, resident) / 8 * 8);
+        T.order == 1 ? 8ll * T.chunk * T.gz : (long long)T.ntiles;
+    if (blocks >= (1ll << 31))
+        return;
     T.nwords = P.nz / 32 + 3;
     unsigned *mask = arena.alloc<unsigned>((size_t)T.ncols * T.nwords);
     {
         const long long words = (long long)T.ncols * T.nwords;
-        hipLaunchKernelGGL(sw::k_zero_u32, dim3((unsigned)std::min<long long>((words + 255) / 256, 4096)),
+        hipLaunchKernelGGL(sw::k_zero_u32,
+                           dim3((unsigned)std::min<long long>((words + 255) / 256, 4096)),
                            dim3(256), 0, st, mask, words);
     }
     T.dmask = mask;
-    tiled_flag_fn(TXR)(P.e, P, T, mask, st);
+    hipLaunchKernelGGL(sw::k_damp_flags, dim3(T.ncols, P.nz), dim3(256), 0, st, P.e, P.nz,
+                       P.nx, P.ny, R, T.gk, T.ty, T.txr, T.nwords, mask);
     hip_check(hipGetLastError(), "k_damp_flags launch");
     Params *Pd = arena.alloc<Params>(1);
     hip_check(hipMemcpyAsync(Pd, &P, sizeof(Params), hipMemcpyHostToDevice, st),
               "copy params");
     pr.Pd = Pd;
-    {
-        const long long npts = pr.npts;
-        real_t *C = arena.alloc<real_t>(npts);
-        real_t *B = arena.alloc<real_t>(npts);
-        double *Aa = arena.alloc<double>(npts);
-        const int nb = (int)std::min<long long>((npts + 255) / 256, 8192);
-        hipLaunchKernelGGL(sw::k_update_coeffs, dim3(nb), dim3(256), 0, st, P.q, P.e, C, B,
-                           Aa, npts);
-        hip_check(hipGetLastError(), "k_update_coeffs launch");
-        pr.cC = C;
-        pr.cB = B;
-        pr.cA = Aa;
-    }
     pr.fastdiv = !env_flag("SIMWAVE_HIP_IEEE_DIV") && make_fixed_div(P.dz2, pr.dvz) &&
                  make_fixed_div(P.dx2, pr.dvx) && make_fixed_div(P.dy2, pr.dvy);
     pr.tiled_launch = K.launch;
@@ -853,6 +832,7 @@ double run_host(const Args &a)
     const double t0 = now_s();
     for (double &t : g_timings)
         t = 0.0;
+    validate(a);
     int ndev = 0;
     hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
     if (ndev < 1)
@@ -970,6 +950,7 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
     const double t0 = now_s();
     for (double &t : g_timings)
         t = 0.0;
+    validate(a);
     keep_pool_memory();
     std::vector<StepSlots> plan = plan_slots(a);
     for (size_t idx = 0; idx < plan.size(); idx++) {

@@ -91,19 +91,6 @@ __device__ __forceinline__ real_t leapfrog(real_t value, real_t q, real_t e,
     return (real_t)t;
 }
 
-// Leap-frog update from per-point coefficients precomputed once per run:
-// C = (dt2/s)/den, B = num/den (working type) and A = 2.0/den (double), the
-// values the reference recomputes at every step (wave.c:177-185).  With
-// e == 0 they are C == q, B == 1, A == 2 exactly.
-__device__ __forceinline__ real_t leapfrog_coef(real_t value, real_t C, real_t B,
-                                                double A, real_t uc, real_t up)
-{
-    value = value * C;
-    double t = A * (double)uc - (double)(B * up);
-    t = t + (double)value;
-    return (real_t)t;
-}
-
 // Correctly rounded a / b for a fixed divisor b (Markstein's theorem):
 // with y = RN(1/b), q0 = RN(a*y), r = a - b*q0 is exact and RN(q0 + r*y) is
 // RN(a/b), provided nothing under- or overflows.  `lo`/`hi` bound |a| so that
@@ -210,22 +197,6 @@ __global__ void k_prepare(const real_t *vel, const real_t *damp,
         const real_t s = (real_t)(1.0 / (double)(v * v));
         q[p] = dt2 / s;
         e[p] = damp[p] * dt / (2 * s);
-    }
-}
-
-// The per-point update coefficients C, B, A from q and e (see leapfrog_coef).
-__global__ void k_update_coeffs(const real_t *__restrict__ q, const real_t *__restrict__ e,
-                                real_t *__restrict__ C, real_t *__restrict__ B,
-                                double *__restrict__ A, long long npts)
-{
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-         p < npts; p += (long long)gridDim.x * blockDim.x) {
-        const real_t ev = e[p];
-        const real_t den = (real_t)(1.0 + (double)ev);
-        const real_t num = (real_t)(1.0 - (double)ev);
-        C[p] = q[p] / den;
-        B[p] = num / den;
-        A[p] = 2.0 / (double)den;
     }
 }
 

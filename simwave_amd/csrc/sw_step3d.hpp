@@ -1,25 +1,29 @@
-// Tiled, software-pipelined 3D step kernel for gfx950.
+// Tiled 3D step kernel for gfx950 with 16-byte lanes.
 //
 // One step of the reference time loop (stencil, sources, boundaries; wave.c
-// sections 1-3) over the interior, fused in one launch.  Work unit = tile:
-// 64 points along the contiguous axis y (one wavefront row) x TXR = 4*RPT rows
-// along x x zlen planes along z.  A persistent grid walks the tiles; the
-// blocks resident on one XCD (blockIdx % 8) take a contiguous range of tiles
-// so neighbouring tiles share that XCD's L2.  Inside a tile each thread owns
-// RPT x-rows of one y column and marches along z:
-//   * z neighbours: a register queue of 2R+1 values per row;
-//   * x/y neighbours: the current plane in LDS with an R-wide halo, double
-//     buffered so one barrier per plane suffices;
-//   * the next plane's centre values, halo, u_prev, q and the damping term are
-//     loaded while the current plane is computed;
-//   * every global access is a buffer instruction: an SGPR descriptor rebased
-//     at the tile's first plane, a per-plane SGPR offset and a per-thread
-//     VGPR offset fixed for the whole tile, so the z march needs no vector
-//     address arithmetic.  Offsets of lanes outside the grid are clamped to a
-//     valid element, so no load is predicated.
-// Per-(tile, plane) flags mark planes without damping (their damping terms
-// are not read and the update needs no division), and tile-uniform tests
-// keep the boundary and source code out of interior tiles.
+// sections 1-3) over the interior, fused in one launch.  Every lane owns V
+// consecutive points of the contiguous axis y (V = 4 floats or 2 doubles, one
+// 16-byte buffer access); a wavefront covers WR rows of 64/WR lanes, so a tile
+// row is TY = 64/WR * V points wide.  A work-group of NW wavefronts owns a tile
+// of TXR = NW*WR*RPT rows along x and marches it along z over zlen planes:
+//   * z neighbours: a per-lane register queue of 2R+1 vectors per row;
+//   * x/y neighbours: the current plane in LDS with an R-wide halo (rounded
+//     up to whole vectors), double buffered so one barrier per plane suffices;
+//   * the next plane's centre values, halos, u_prev, q and (on planes that
+//     hold damping) the damping term are loaded while the current plane is
+//     computed;
+//   * every global access is a buffer instruction on a descriptor rebased at
+//     the tile's first plane.  The z-queue loads carry the plane offset in the
+//     VGPR offset, so the descriptor's range check (which covers the VGPR
+//     offset) turns the only possible overrun, the tail vector of the array's
+//     last row, into zeros; the other accesses stay inside the array by
+//     construction and use an SGPR plane offset.
+// Per-(tile column, plane) bits mark planes without damping: there the update
+// needs no division and the damping term is not read.  Tile-uniform tests
+// keep the boundary and source code out of interior tiles.  Tiles are visited
+// column-fastest (all columns of one z range before the next range); with
+// order 1 the columns are cut into 8 contiguous chunks, one per XCD (blocks
+// are dispatched to XCDs round-robin), so tiles sharing halo rows share an L2.
 #pragma once
 #include "sw_kernels.hpp"
 
@@ -27,11 +31,23 @@ namespace sw {
 
 #if SW_DIM == 3
 
+#if defined(SW_DOUBLE)
+constexpr int kVec = 2;
+typedef double vreal __attribute__((ext_vector_type(2)));
+#else
+constexpr int kVec = 4;
+typedef float vreal __attribute__((ext_vector_type(4)));
+#endif
+typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
+
 struct Tiling {
-    int gk, gj, gz;     // tiles along y (64 wide), x (TXR rows), z (zlen planes)
+    int gk, gj, gz;     // tiles along y (ty wide), x (txr rows), z (zlen planes)
+    int ty, txr;        // tile width along y and rows along x
     int zlen;
     int ntiles;
     int ncols;          // gk * gj
+    int order;          // 0 column-fastest, 1 column-fastest in 8 XCD chunks
+    int chunk;          // columns per XCD chunk (order 1)
     int nwords;         // 32-bit mask words per tile column (nz/32 + 3)
     const unsigned *dmask;  // [ncols][nwords]: bit z set where plane z of the
                             // tile column holds a non-zero damping term
@@ -62,17 +78,16 @@ __device__ __forceinline__ real_t add_sources_masked(const Params &P, int n, int
 
 // Marks tile planes holding a non-zero damping term.  grid (ncols, nz); the
 // mask words must be zeroed beforehand.
-template <int TXR>
 __global__ void __launch_bounds__(256)
 k_damp_flags(const real_t *__restrict__ e, int nz, int nx, int ny, int R, int gk,
-             int nwords, unsigned *__restrict__ mask)
+             int ty, int txr, int nwords, unsigned *__restrict__ mask)
 {
     const int col = blockIdx.x, z = blockIdx.y;
     const int kt = col % gk, jt = col / gk;
-    const int k0 = R + kt * 64, j0 = R + jt * TXR;
+    const int k0 = R + kt * ty, j0 = R + jt * txr;
     int any = 0;
-    for (int t = threadIdx.x; t < 64 * TXR; t += blockDim.x) {
-        const int k = k0 + (t & 63), j = j0 + (t >> 6);
+    for (int t = threadIdx.x; t < ty * txr; t += blockDim.x) {
+        const int k = k0 + t % ty, j = j0 + t / ty;
         if (k < ny && j < nx)
             any |= e[((long long)z * nx + j) * ny + k] != (real_t)0;
     }
@@ -101,22 +116,18 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *base, unsigned long long
                                              0x00020000);
 }
 
-// Buffer loads/stores move raw bits; the values are bit-cast, not converted.
-__device__ __forceinline__ real_t bload(rsrc_t r, unsigned voff, unsigned soff)
+// 16-byte buffer accesses; the values are bit-cast, not converted.
+__device__ __forceinline__ vreal vload(rsrc_t r, unsigned voff, unsigned soff)
 {
-#if defined(SW_DOUBLE)
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-#else
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-#endif
+    return __builtin_bit_cast(vreal, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-__device__ __forceinline__ double bload_f64(rsrc_t r, unsigned voff, unsigned soff)
+__device__ __forceinline__ void vstore(vreal v, rsrc_t r, unsigned voff, unsigned soff)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), r, voff, soff, 0);
 }
 
-__device__ __forceinline__ void bstore(real_t v, rsrc_t r, unsigned voff, unsigned soff)
+__device__ __forceinline__ void sstore(real_t v, rsrc_t r, unsigned voff, unsigned soff)
 {
 #if defined(SW_DOUBLE)
     typedef unsigned v2u32 __attribute__((ext_vector_type(2)));
@@ -126,11 +137,35 @@ __device__ __forceinline__ void bstore(real_t v, rsrc_t r, unsigned voff, unsign
 #endif
 }
 
+// Leap-frog update on a plane holding damping: the reference's full
+// expression (wave.c:177-185).  With e == 0 it gives the same bits as
+// leapfrog_plain, so one form serves every point of such a plane.
+__device__ __forceinline__ real_t leapfrog_damped(real_t value, real_t q, real_t e,
+                                                  real_t uc, real_t up)
+{
+    const real_t den = (real_t)(1.0 + (double)e);
+    const real_t num = (real_t)(1.0 - (double)e);
+    value = value * (q / den);
+    const double a = 2.0 / (double)den;
+    const real_t b = num / den;
+    double t = a * (double)uc - (double)(b * up);
+    t = t + (double)value;
+    return (real_t)t;
+}
+
+// Leap-frog update where den == num == 1 (no damping at the point).
+__device__ __forceinline__ real_t leapfrog_plain(real_t value, real_t q, real_t uc,
+                                                 real_t up)
+{
+    value = value * q;
+    double t = 2.0 * (double)uc - (double)up;
+    t = t + (double)value;
+    return (real_t)t;
+}
+
 // The kernel's run constants, kept small so they stay in SGPRs.
-struct TiledArgs {
-    const real_t *uc, *up;
-    const real_t *C, *B;   // update coefficients (working type)
-    const double *A;       // 2.0/den
+struct StepArgs {
+    const real_t *uc, *up, *q, *e;
     real_t *un;
     int nz, nx, ny;
     unsigned pbytes;    // bytes per z plane (working type)
@@ -145,244 +180,310 @@ struct TiledArgs {
     Tiling T;
 };
 
-template <int R, int RPT, int NW>
-__global__ void __launch_bounds__(64 * NW)
-k_step3d_tiled(TiledArgs A, const Params *__restrict__ Pd, int n)
-{
-    constexpr int TY = 64;
-    constexpr int NT = 64 * NW;
-    constexpr int TXR = NW * RPT;
-    constexpr int LS = 128;                 // LDS row stride (elements)
-    constexpr int H = TXR + 2 * R + 1;      // + one junk row
-    constexpr int JUNK = TXR + 2 * R;
-    constexpr int HX = (2 * R + NW - 1) / NW;
-    constexpr int NYH = TXR * 2 * R;
-    constexpr int HY = (NYH + NT - 1) / NT;
-    constexpr unsigned ascale = sizeof(double) / sizeof(real_t);
-    __shared__ real_t lds[2][H][LS];
+template <int R, int WR, int RPT, int NW>
+struct TileShape {
+    static constexpr int V = kVec;
+    static constexpr int LW = 64 / WR;                 // lanes per row
+    static constexpr int TY = LW * V;                  // tile width along y
+    static constexpr int TXR = NW * WR * RPT;          // tile rows along x
+    static constexpr int PADL = (R + V - 1) / V * V;   // y halo, whole vectors
+    static constexpr int HV = PADL / V;                // halo vectors per side
+    static constexpr int LS = TY + 2 * PADL;           // LDS row stride
+    static constexpr int H = TXR + 2 * R;              // LDS rows
+    static constexpr int SLOTS = NW * WR;              // rows one instruction covers
+    static constexpr int HX = (2 * R + SLOTS - 1) / SLOTS;
+    static constexpr int NYV = TXR * 2 * HV;           // y-halo vectors per plane
+    static constexpr int HY = (NYV + 64 * NW - 1) / (64 * NW);
+};
 
-    const int tx = threadIdx.x, w = threadIdx.y;
-    const int tid = w * TY + tx;
+template <int R, int WR, int RPT, int NW>
+__global__ void __launch_bounds__(64 * NW)
+k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
+{
+    using S = TileShape<R, WR, RPT, NW>;
+    constexpr int V = S::V, LW = S::LW, TY = S::TY, TXR = S::TXR, PADL = S::PADL;
+    constexpr int HV = S::HV, LS = S::LS, H = S::H, SLOTS = S::SLOTS, HX = S::HX;
+    constexpr int NYV = S::NYV, HY = S::HY;
+    constexpr int NT = 64 * NW;
+    constexpr unsigned esz = sizeof(real_t);
+    __shared__ __align__(16) real_t lds[2][H][LS];
+
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int tid = w * 64 + lane;
+    const int slot = w * WR + lane / LW;     // row slot of this lane
+    const int lc = lane % LW;                // vector column of this lane
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const unsigned PB = A.pbytes;
-    const unsigned esz = sizeof(real_t);
     const Tiling &T = A.T;
 
-    const int nb8 = gridDim.x >> 3;
-    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3;
-    const int t_begin = (int)(((long long)xcd * T.ntiles) >> 3);
-    const int t_end = (int)(((long long)(xcd + 1) * T.ntiles) >> 3);
+    int col, zc;
+    if (T.order == 1) {
+        const int xcd = blockIdx.x & 7, s = blockIdx.x >> 3;
+        zc = s / T.chunk;
+        col = xcd * T.chunk + s % T.chunk;
+        if (col >= T.ncols)
+            return;
+    } else {
+        col = blockIdx.x % T.ncols;
+        zc = blockIdx.x / T.ncols;
+    }
+    const int kt = col % T.gk, jt = col / T.gk;
+    const int k0 = R + kt * TY, j0 = R + jt * TXR;
+    const int z0 = R + zc * T.zlen;
+    const int z1 = min(z0 + T.zlen, nz - R);
+    const int zb = z0 - R;                  // first plane the tile touches
 
-    for (int t = t_begin + lb; t < t_end; t += nb8) {
-        const int zc = t % T.gz, col = t / T.gz;
-        const int kt = col % T.gk, jt = col / T.gk;
-        const int k0 = R + kt * TY, j0 = R + jt * TXR;
-        const int z0 = R + zc * T.zlen;
-        const int z1 = min(z0 + T.zlen, nz - R);
-        const int zb = z0 - R;              // first plane the tile touches
-        const int k = k0 + tx;
+    // descriptors rebased at plane zb
+    const unsigned long long off0 = (unsigned long long)zb * PB;
+    const unsigned long long rem = (unsigned long long)(nz - zb) * PB;
+    const rsrc_t r_uc = make_rsrc(A.uc + off0 / esz, rem);
+    const rsrc_t r_up = make_rsrc(A.up + off0 / esz, rem);
+    const rsrc_t r_q = make_rsrc(A.q + off0 / esz, rem);
+    const rsrc_t r_e = make_rsrc(A.e + off0 / esz, rem);
+    const rsrc_t r_un = make_rsrc(A.un + off0 / esz, rem);
 
-        // descriptors rebased at plane zb
-        const unsigned long long off0 = (unsigned long long)zb * PB;
-        const unsigned long long rem = (unsigned long long)(nz - zb) * PB;
-        const rsrc_t r_uc = make_rsrc(A.uc + off0 / esz, rem);
-        const rsrc_t r_up = make_rsrc(A.up + off0 / esz, rem);
-        const rsrc_t r_C = make_rsrc(A.C + off0 / esz, rem);
-        const rsrc_t r_B = make_rsrc(A.B + off0 / esz, rem);
-        const rsrc_t r_A = make_rsrc(A.A + off0 / esz, rem * ascale);
-        const rsrc_t r_un = make_rsrc(A.un + off0 / esz, rem);
+    // This lane's points: rows j0 + slot + SLOTS*q, columns k .. k+V-1.
+    // Lanes past the row end read the row start (values unused); a lane
+    // straddling the row end reads at most V-1 elements of the next row.
+    const int k = k0 + lc * V;
+    const int kl = k < ny ? k : 0;
+    unsigned cb[RPT];
+    bool act[RPT];      // row computed
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int j = j0 + slot + SLOTS * q;
+        act[q] = j < nx - R;
+        const int jl = j < nx ? j : nx - 1;
+        cb[q] = ((unsigned)jl * (unsigned)ny + (unsigned)kl) * esz;
+    }
+    const bool full = k + V <= ny - R;      // every point of the vector is interior
+    // x halo: rows j0-R .. j0-1 and j0+TXR .. j0+TXR+R-1
+    unsigned xh_off[HX];
+    int xh_row[HX];
+#pragma unroll
+    for (int h = 0; h < HX; h++) {
+        const int hr = slot + SLOTS * h;
+        const int jr = hr < R ? j0 - R + hr : j0 + TXR + (hr - R);
+        const int jl = jr < nx ? jr : nx - 1;
+        xh_off[h] = ((unsigned)jl * (unsigned)ny + (unsigned)kl) * esz;
+        xh_row[h] = hr < 2 * R ? (hr < R ? hr : hr + TXR) : -1;
+    }
+    // y halo: HV vectors left of k0 and right of k0+TY on every tile row
+    unsigned yh_off[HY];
+    int yh_lds[HY];
+#pragma unroll
+    for (int h = 0; h < HY; h++) {
+        const int idx = tid + NT * h;
+        const int row = idx / (2 * HV), c = idx % (2 * HV);
+        const int jr = j0 + row;
+        const int jl = jr < nx ? jr : nx - 1;
+        int kr = c < HV ? k0 - PADL + V * c : k0 + TY + V * (c - HV);
+        kr = kr < ny ? kr : 0;
+        // kr > -V and jl >= R >= 1 with ny >= V, so the offset is not negative
+        yh_off[h] = (unsigned)((int)((unsigned)jl * (unsigned)ny) + kr) * esz;
+        yh_lds[h] = idx < NYV ? (R + row) * LS + (c < HV ? V * c : PADL + TY + V * (c - HV))
+                              : -1;
+    }
 
-        unsigned cb[RPT];
-        bool act[RPT];
+    // tile-uniform boundary and source tests
+    const int bc = A.bc;
+    const int r2 = 2 * R;
+    bool bc_xy = false;
+    if ((bc >> 8 & 3) && k0 <= r2)
+        bc_xy = true;
+    if ((bc >> 10 & 3) && k0 + TY - 1 >= ny - r2 - 1)
+        bc_xy = true;
+    if ((bc >> 4 & 3) && j0 <= r2)
+        bc_xy = true;
+    if ((bc >> 6 & 3) && j0 + TXR - 1 >= nx - r2 - 1)
+        bc_xy = true;
+    const bool bz_b = (bc & 3) != 0, bz_a = (bc >> 2 & 3) != 0;
+    unsigned long long smask = 0;
+    for (int s = 0; s < A.nsrc; s++) {
+        const Box b = A.sbox[s];
+        if (b.hi[1] >= j0 && b.lo[1] < j0 + TXR && b.hi[2] >= k0 && b.lo[2] < k0 + TY)
+            smask |= 1ull << s;
+    }
+
+    // prologue: z queue planes zb .. zb+2R-1 (plane offset in the VGPR
+    // offset: range checked against the array end)
+    vreal qz[RPT][2 * R + 1];
+#pragma unroll
+    for (int m = 0; m < 2 * R; m++)
+#pragma unroll
+        for (int q = 0; q < RPT; q++)
+            qz[q][m + 1] = vload(r_uc, cb[q] + (unsigned)m * PB, 0u);
+    // One register set per prefetched quantity: the centre plane and its
+    // halos for plane i+1 are loaded right after plane i is written to LDS;
+    // u_prev, q and the damping term for plane i+1 right after plane i's
+    // update has consumed them.
+    vreal nc[RPT], np[RPT], nq[RPT], ne[RPT], nhx[HX], nhy[HY];
+    const unsigned long long dbits = damp_bits(T, col, z0);
+    auto load_centre = [&](int I) {
+        const unsigned sp = (unsigned)(I - zb) * PB;   // plane I
+        const unsigned vc = (unsigned)(I + R - zb) * PB;
+#pragma unroll
+        for (int q = 0; q < RPT; q++)
+            nc[q] = vload(r_uc, cb[q] + vc, 0u);
+#pragma unroll
+        for (int h = 0; h < HX; h++)
+            nhx[h] = vload(r_uc, xh_off[h], sp);
+#pragma unroll
+        for (int h = 0; h < HY; h++)
+            nhy[h] = vload(r_uc, yh_off[h], sp);
+    };
+    auto load_update = [&](int I) {
+        const unsigned sp = (unsigned)(I - zb) * PB;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
-            const int j = j0 + w + NW * q;
-            const bool in = (k < ny) && (j < nx);
-            act[q] = (k < ny - R) && (j < nx - R);
-            cb[q] = in ? ((unsigned)j * (unsigned)ny + (unsigned)k) * esz : 0u;
+            np[q] = vload(r_up, cb[q], sp);
+            nq[q] = vload(r_q, cb[q], sp);
         }
-        unsigned xh_off[HX];
-        int xh_lds[HX];
-#pragma unroll
-        for (int h = 0; h < HX; h++) {
-            const int hr = w + NW * h;
-            const int jr = hr < R ? j0 - R + hr : j0 + TXR + (hr - R);
-            const bool ok = hr < 2 * R && k < ny && jr < nx;
-            xh_off[h] = ok ? ((unsigned)jr * (unsigned)ny + (unsigned)k) * esz : 0u;
-            xh_lds[h] = (ok ? (hr < R ? hr : hr + TXR) : JUNK) * LS + R + tx;
-        }
-        unsigned yh_off[HY];
-        int yh_lds[HY];
-#pragma unroll
-        for (int h = 0; h < HY; h++) {
-            const int idx = tid + NT * h;
-            const int row = idx / (2 * R), c = idx % (2 * R);
-            const int jr = j0 + row;
-            const int kr = c < R ? k0 - R + c : k0 + TY + (c - R);
-            const bool ok = idx < NYH && jr < nx && kr < ny;
-            yh_off[h] = ok ? ((unsigned)jr * (unsigned)ny + (unsigned)kr) * esz : 0u;
-            yh_lds[h] = ok ? (R + row) * LS + (c < R ? c : c + TY) : JUNK * LS + tx;
-        }
-
-        // tile-uniform boundary and source tests
-        const int bc = A.bc;
-        const int r2 = 2 * R;
-        bool bc_xy = false;
-        if ((bc >> 8 & 3) && k0 <= r2)
-            bc_xy = true;
-        if ((bc >> 10 & 3) && k0 + TY - 1 >= ny - r2 - 1)
-            bc_xy = true;
-        if ((bc >> 4 & 3) && j0 <= r2)
-            bc_xy = true;
-        if ((bc >> 6 & 3) && j0 + TXR - 1 >= nx - r2 - 1)
-            bc_xy = true;
-        const bool bz_b = (bc & 3) != 0, bz_a = (bc >> 2 & 3) != 0;
-        unsigned long long smask = 0;
-        for (int s = 0; s < A.nsrc; s++) {
-            const Box b = A.sbox[s];
-            if (b.hi[1] >= j0 && b.lo[1] < j0 + TXR && b.hi[2] >= k0 &&
-                b.lo[2] < k0 + TY)
-                smask |= 1ull << s;
-        }
-
-        // prologue: z queue planes zb .. zb+2R-1 and the first plane
-        real_t qz[RPT][2 * R + 1];
-#pragma unroll
-        for (int m = 0; m < 2 * R; m++)
+        if ((dbits >> (I - z0)) & 1) {
 #pragma unroll
             for (int q = 0; q < RPT; q++)
-                qz[q][m + 1] = bload(r_uc, cb[q], (unsigned)m * PB);
-        real_t nc[RPT], np[RPT], nC[RPT], nB[RPT], nhx[HX], nhy[HY];
-        double nA[RPT];
-        const unsigned long long dbits = damp_bits(T, col, z0);
-        auto load_plane = [&](int I) {
-            const unsigned sp = (unsigned)(I - zb) * PB;   // plane I
-            const unsigned sc = sp + (unsigned)R * PB;     // plane I+R
-#pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                nc[q] = bload(r_uc, cb[q], sc);
-                np[q] = bload(r_up, cb[q], sp);
-                nC[q] = bload(r_C, cb[q], sp);
-            }
-            if ((dbits >> (I - z0)) & 1) {
-#pragma unroll
-                for (int q = 0; q < RPT; q++) {
-                    nB[q] = bload(r_B, cb[q], sp);
-                    nA[q] = bload_f64(r_A, cb[q] * ascale, sp * ascale);
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < RPT; q++) {
-                    nB[q] = (real_t)1;
-                    nA[q] = 2.0;
-                }
-            }
-#pragma unroll
-            for (int h = 0; h < HX; h++)
-                nhx[h] = bload(r_uc, xh_off[h], sp);
-#pragma unroll
-            for (int h = 0; h < HY; h++)
-                nhy[h] = bload(r_uc, yh_off[h], sp);
-        };
-        load_plane(z0);
-
-        // Stores of plane i are issued in iteration i+1, ahead of that
-        // iteration's prefetch: vmcnt counts stores too, so a store issued
-        // after the prefetch would make the next loop-top wait cover it.
-        real_t pend[RPT];
-        bool have_pend = false;
-        unsigned pend_so = 0;
-
-        for (int i = z0; i < z1; i++) {
-            real_t *lb_ = &lds[(i - z0) & 1][0][0];
-            real_t cp[RPT], cC[RPT], cB[RPT];
-            double cA[RPT];
-#pragma unroll
-            for (int q = 0; q < RPT; q++) {
-#pragma unroll
-                for (int m = 0; m < 2 * R; m++)
-                    qz[q][m] = qz[q][m + 1];
-                qz[q][2 * R] = nc[q];
-                cp[q] = np[q];
-                cC[q] = nC[q];
-                cB[q] = nB[q];
-                cA[q] = nA[q];
-                lb_[(R + w + NW * q) * LS + R + tx] = qz[q][R];
-            }
-#pragma unroll
-            for (int h = 0; h < HX; h++)
-                lb_[xh_lds[h]] = nhx[h];
-#pragma unroll
-            for (int h = 0; h < HY; h++)
-                lb_[yh_lds[h]] = nhy[h];
-            if (have_pend) {
-#pragma unroll
-                for (int q = 0; q < RPT; q++)
-                    if (act[q])
-                        bstore(pend[q], r_un, cb[q], pend_so);
-            }
-            if (i + 1 < z1)
-                load_plane(i + 1);
-            __syncthreads();
-
-            const bool bc_here = bc_xy || (bz_b && i <= r2) || (bz_a && i >= nz - r2 - 1);
-            const bool special = (smask != 0) | bc_here;
-            const unsigned so = (unsigned)(i - zb) * PB;
-#pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                // lowest address this point reads: (row - R, column - R)
-                const real_t *lo = lb_ + (w + NW * q) * LS + tx;
-                const real_t c = qz[q][R];
-                real_t acc_c = A.c2[0] * c;
-                real_t acc_m = A.c2[0] * c;
-                real_t acc_z = A.c2[0] * c;
-#pragma unroll
-                for (int m = 1; m <= R; m++) {
-                    acc_c += A.c2[m] * (lo[R * LS + R + m] + lo[R * LS + R - m]);
-                    acc_m += A.c2[m] * (lo[(R + m) * LS + R] + lo[(R - m) * LS + R]);
-                    acc_z += A.c2[m] * (qz[q][R + m] + qz[q][R - m]);
-                }
-                // fixed-divisor path when every numerator is in the safe
-                // range or all are zero; IEEE division otherwise
-                const real_t mx = fmax(fmax(fabs(acc_c), fabs(acc_m)), fabs(acc_z));
-                const real_t mn = fmin(fmin(fabs(acc_c), fabs(acc_m)), fabs(acc_z));
-                const bool fast = A.fastdiv & (((mn >= A.div_lo) & (mx <= A.div_hi)) |
-                                               (mx == (real_t)0));
-                real_t lap;
-                if (fast)
-                    lap = fixed_div(acc_c, A.dvy) + fixed_div(acc_m, A.dvx) +
-                          fixed_div(acc_z, A.dvz);
-                else
-                    lap = acc_c / A.dy2 + acc_m / A.dx2 + acc_z / A.dz2;
-                real_t value = 0;
-                value += lap;
-                real_t v = leapfrog_coef(value, cC[q], cB[q], cA[q], c, cp[q]);
-                if (special) {
-                    // tiles touching a source box or a boundary face: store now
-                    if (!act[q])
-                        continue;
-                    const int j = j0 + w + NW * q;
-                    const long long p = (long long)i * nx * ny + (long long)j * ny + k;
-                    if (smask)
-                        v = add_sources_masked(*Pd, n, i, j, k, Pd->q[p], Pd->e[p], v, smask);
-                    if (bc_here)
-                        store_with_bc(*Pd, A.un, p, i, j, k, v);
-                    else
-                        bstore(v, r_un, cb[q], so);
-                } else {
-                    pend[q] = v;
-                }
-            }
-            have_pend = !special;
-            pend_so = so;
+                ne[q] = vload(r_e, cb[q], sp);
         }
-        if (have_pend) {
+    };
+    load_centre(z0);
+    load_update(z0);
+
+    for (int i = z0; i < z1; i++) {
+        real_t *lb = &lds[(i - z0) & 1][0][0];
+        const bool damped = (dbits >> (i - z0)) & 1;
 #pragma unroll
-            for (int q = 0; q < RPT; q++)
-                if (act[q])
-                    bstore(pend[q], r_un, cb[q], pend_so);
+        for (int q = 0; q < RPT; q++) {
+#pragma unroll
+            for (int m = 0; m < 2 * R; m++)
+                qz[q][m] = qz[q][m + 1];
+            qz[q][2 * R] = nc[q];
+            *(vreal *)&lb[(R + slot + SLOTS * q) * LS + PADL + lc * V] = qz[q][R];
         }
+#pragma unroll
+        for (int h = 0; h < HX; h++)
+            if (xh_row[h] >= 0)
+                *(vreal *)&lb[xh_row[h] * LS + PADL + lc * V] = nhx[h];
+#pragma unroll
+        for (int h = 0; h < HY; h++)
+            if (yh_lds[h] >= 0)
+                *(vreal *)&lb[yh_lds[h]] = nhy[h];
+        const bool more = i + 1 < z1;
+        if (more)
+            load_centre(i + 1);
         __syncthreads();
+
+        const bool bc_here = bc_xy || (bz_b && i <= r2) || (bz_a && i >= nz - r2 - 1);
+        const bool special = (smask != 0) | bc_here;
+        const unsigned so = (unsigned)(i - zb) * PB;
+        vreal out[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int row = R + slot + SLOTS * q;
+            const real_t *lrow = lb + row * LS + lc * V;   // y window start
+            // y window: points k-PADL .. k+V+PADL-1
+            real_t win[V + 2 * PADL];
+#pragma unroll
+            for (int h = 0; h < HV; h++) {
+                const vreal a = *(const vreal *)&lrow[V * h];
+                const vreal b = *(const vreal *)&lrow[PADL + V + V * h];
+#pragma unroll
+                for (int e = 0; e < V; e++) {
+                    win[V * h + e] = a[e];
+                    win[PADL + V + V * h + e] = b[e];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < V; e++)
+                win[PADL + e] = qz[q][R][e];
+            real_t acc_y[V], acc_x[V], acc_z[V];
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                const real_t c0 = A.c2[0] * qz[q][R][e];
+                acc_y[e] = c0;
+                acc_x[e] = c0;
+                acc_z[e] = c0;
+            }
+#pragma unroll
+            for (int m = 1; m <= R; m++) {
+                const vreal xp = *(const vreal *)&lrow[m * LS + PADL];
+                const vreal xm = *(const vreal *)&lrow[-m * LS + PADL];
+#pragma unroll
+                for (int e = 0; e < V; e++) {
+                    acc_y[e] += A.c2[m] * (win[PADL + e + m] + win[PADL + e - m]);
+                    acc_x[e] += A.c2[m] * (xp[e] + xm[e]);
+                    acc_z[e] += A.c2[m] * (qz[q][R + m][e] + qz[q][R - m][e]);
+                }
+            }
+            // Fixed-divisor quotients for every point; the IEEE quotients
+            // replace them where a numerator is outside the safe range (or
+            // zero next to a non-zero one), decided once per wavefront.
+            real_t lap[V];
+            bool ok = true;
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                const real_t ay = acc_y[e], ax = acc_x[e], az = acc_z[e];
+                const real_t mx = fmax(fmax(fabs(ay), fabs(ax)), fabs(az));
+                const real_t mn = fmin(fmin(fabs(ay), fabs(ax)), fabs(az));
+                ok &= ((mn >= A.div_lo) & (mx <= A.div_hi)) | (mx == (real_t)0);
+                lap[e] = fixed_div(ay, A.dvy) + fixed_div(ax, A.dvx) + fixed_div(az, A.dvz);
+            }
+            if (!__all(ok & (A.fastdiv != 0))) {
+#pragma unroll
+                for (int e = 0; e < V; e++) {
+                    const real_t ay = acc_y[e], ax = acc_x[e], az = acc_z[e];
+                    const real_t mx = fmax(fmax(fabs(ay), fabs(ax)), fabs(az));
+                    const real_t mn = fmin(fmin(fabs(ay), fabs(ax)), fabs(az));
+                    const bool fe = A.fastdiv &
+                                    (((mn >= A.div_lo) & (mx <= A.div_hi)) | (mx == (real_t)0));
+                    if (!fe)
+                        lap[e] = ay / A.dy2 + ax / A.dx2 + az / A.dz2;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                real_t value = 0;
+                value += lap[e];
+                out[q][e] = damped ? leapfrog_damped(value, nq[q][e], ne[q][e], qz[q][R][e],
+                                                     np[q][e])
+                                   : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
+            }
+            if (special && act[q]) {
+                // tiles touching a source box or a boundary face
+                const int j = j0 + slot + SLOTS * q;
+#pragma unroll
+                for (int e = 0; e < V; e++) {
+                    const int kk = k + e;
+                    if (kk >= ny - R)
+                        continue;
+                    const long long p = (long long)i * nx * ny + (long long)j * ny + kk;
+                    real_t v = out[q][e];
+                    if (smask)
+                        v = add_sources_masked(*Pd, n, i, j, kk, nq[q][e],
+                                               damped ? ne[q][e] : (real_t)0, v, smask);
+                    if (bc_here)
+                        store_with_bc(*Pd, A.un, p, i, j, kk, v);
+                    else
+                        sstore(v, r_un, cb[q] + e * esz, so);
+                }
+            }
+        }
+        if (more)
+            load_update(i + 1);
+        if (!special) {
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                if (!act[q])
+                    continue;
+                if (full) {
+                    vstore(out[q], r_un, cb[q], so);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; e++)
+                        if (k + e < ny - R)
+                            sstore(out[q][e], r_un, cb[q] + e * esz, so);
+                }
+            }
+        }
     }
 }
 

@@ -100,18 +100,21 @@ def readme_2d():
     return setup
 
 
-def golden_2d(space_order):
+def golden_2d(space_order, density=False):
     """Configuration of the reference's golden 2D wavefields
-    (tests/reference_solution/generator.py:6-50 of the reference)."""
+    (tests/reference_solution/generator.py:6-50 of the reference); with
+    density=True the unit-density variable-density run its tests compare
+    against the same wavefield (tests/test_solution.py:68-81)."""
 
     def setup():
         shape = (500, 500)
         vel = np.zeros(shape=shape, dtype=np.float32)
         vel[:] = 1500.0
         vel[shape[0] // 2:] = 2000.0
+        den = np.ones(shape, dtype=np.float32) if density else None
         sm = sw.SpaceModel(bounding_box=(0, 5000, 0, 5000), grid_spacing=(10, 10),
-                           velocity_model=vel, space_order=space_order,
-                           dtype=np.float32)
+                           velocity_model=vel, density_model=den,
+                           space_order=space_order, dtype=np.float32)
         sm.config_boundary(damping_length=100,
                            boundary_condition=("null_neumann", "null_dirichlet",
                                                "none", "null_dirichlet"),
@@ -153,3 +156,97 @@ def golden_3d(space_order, density=False):
                     receivers=receiver, wavelet=sw.RickerWavelet(15.0, tm))
 
     return setup
+
+
+def saving_2d(saving_stride, density=False):
+    """The reference's snapshot-stride experiment in 2D
+    (tests/test_u_saving.py:7-71 of the reference)."""
+
+    def setup():
+        vel = np.full((512, 512), 1500.0, dtype=np.float32)
+        vel[250:] = 3000.0
+        den = np.full((512, 512), 5.0, dtype=np.float32) if density else None
+        sm = sw.SpaceModel(bounding_box=(0, 5120, 0, 5120), grid_spacing=(10, 10),
+                           velocity_model=vel, density_model=den, space_order=4,
+                           dtype=np.float32)
+        sm.config_boundary(damping_length=0,
+                           boundary_condition=("null_neumann", "null_dirichlet",
+                                               "none", "null_dirichlet"),
+                           damping_polynomial_degree=3, damping_alpha=0.001)
+        tm = sw.TimeModel(space_model=sm, tf=1.0, saving_stride=saving_stride)
+        source = sw.Source(sm, coordinates=[(2560, 2560)], window_radius=4)
+        receiver = sw.Receiver(sm, coordinates=[(2560, i) for i in range(0, 5120, 10)],
+                               window_radius=4)
+        return dict(space_model=sm, time_model=tm, sources=source,
+                    receivers=receiver, wavelet=sw.RickerWavelet(10.0, tm))
+
+    return setup
+
+
+def saving_3d(saving_stride, density=False):
+    """The reference's snapshot-stride experiment in 3D
+    (tests/test_u_saving.py:74-143 of the reference)."""
+
+    def setup():
+        vel = np.full((100, 100, 100), 1500.0, dtype=np.float32)
+        den = np.full((100, 100, 100), 5.0, dtype=np.float32) if density else None
+        sm = sw.SpaceModel(bounding_box=(0, 1000, 0, 1000, 0, 1000),
+                           grid_spacing=(10, 10, 10), velocity_model=vel,
+                           density_model=den, space_order=4, dtype=np.float32)
+        sm.config_boundary(damping_length=0,
+                           boundary_condition=("null_neumann", "null_dirichlet",
+                                               "null_dirichlet", "null_dirichlet",
+                                               "null_dirichlet", "null_dirichlet"),
+                           damping_polynomial_degree=3, damping_alpha=0.001)
+        tm = sw.TimeModel(space_model=sm, tf=0.4, saving_stride=saving_stride)
+        source = sw.Source(sm, coordinates=[(500, 500, 500)], window_radius=4)
+        receiver = sw.Receiver(sm, coordinates=[(500, 500, i) for i in range(0, 1000, 10)],
+                               window_radius=4)
+        return dict(space_model=sm, time_model=tm, sources=source,
+                    receivers=receiver, wavelet=sw.RickerWavelet(15.0, tm))
+
+    return setup
+
+
+def hankel_accuracy_case(space_order=8, spacing=0.5, dt=0.1, tf=150.075):
+    """The reference's analytic-accuracy experiment: 2D homogeneous 1.5 km/s,
+    0.09 kHz Ricker at (200, 200), one receiver at (260, 260)
+    (tests/test_convergence.py:9-87 of the reference)."""
+
+    def setup():
+        sm = sw.SpaceModel(bounding_box=(-40, 440, -40, 440),
+                           grid_spacing=(spacing, spacing),
+                           velocity_model=1.5 * np.ones((100, 100), dtype=np.float64),
+                           space_order=space_order)
+        tm = sw.TimeModel(space_model=sm, t0=0, tf=tf)
+        tm.dt = dt
+        source = sw.Source(sm, coordinates=[(200, 200)])
+        receiver = sw.Receiver(sm, coordinates=[(260, 260)])
+        return dict(space_model=sm, time_model=tm, sources=source,
+                    receivers=receiver, wavelet=sw.RickerWavelet(0.09, tm))
+
+    return setup
+
+
+def hankel_reference(setup, freq=0.09, src=(200.0, 200.0), rec=(260.0, 260.0)):
+    """Analytic 2D Green's function response to the Ricker wavelet (the
+    zeroth-order Hankel function of the second kind, evaluated in the
+    frequency domain), as in tests/test_convergence.py:27-49 of the
+    reference.  Returns the trace over the run's time steps."""
+    from scipy.special import hankel2
+    parts = setup()
+    sm, tm0 = parts["space_model"], parts["time_model"]
+    tm = sw.TimeModel(space_model=sm, t0=0, tf=3000)
+    tm.dt = tm0.dt
+    wav = sw.RickerWavelet(freq, tm).values
+    nt = tm.timesteps
+    nf = int(nt / 2 + 1)
+    df = 1 / tm.tf
+    f = df * np.arange(nf)
+    qf = np.fft.fft(wav)[:nf]
+    r = np.hypot(rec[0] - src[0], rec[1] - src[1])
+    k = 2 * np.pi * f / np.unique(sm.velocity_model)
+    g = np.zeros(nf, dtype=complex)
+    g[1:-1] = hankel2(0, k[1:-1] * r)
+    trace = np.real(np.fft.ifft(-1j * np.pi * g * qf, nt)) / (2 * np.pi)
+    return trace[:tm0.timesteps]

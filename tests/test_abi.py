@@ -1,0 +1,117 @@
+"""The C ABI: every entry point include/simwave_hip.h declares is exported
+by the library of its variant, the reference FFI symbol `forward` is
+exported by every library, and a call fails loudly (negative return and a
+message, no exit) where it cannot run.  No kernel is launched here."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+import simwave_amd as sw
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "simwave_hip.h")
+
+pytestmark = pytest.mark.skipif(
+    not all(os.path.exists(sw.build.library_path(*v)) for v in sw.build.VARIANTS),
+    reason="HIP libraries not built (python -m simwave_amd.build)")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(simwave_hip_\w+)\s*\(", text)))
+
+
+def test_header_declares_every_variant():
+    names = declared()
+    for v in sw.build.VARIANTS:
+        vn = sw.build.variant_name(*v)
+        for base in ("simwave_hip_forward_", "simwave_hip_forward_device_",
+                     "simwave_hip_fixed_div_"):
+            assert base + vn in names
+    for common in ("simwave_hip_last_error", "simwave_hip_last_timings",
+                   "simwave_hip_variant"):
+        assert common in names
+
+
+def test_header_compiles_as_c():
+    subprocess.run(["gcc", "-fsyntax-only", "-Wall", "-Werror", "-x", "c", HEADER],
+                   check=True)
+
+
+@pytest.mark.parametrize("variant", sw.build.VARIANTS,
+                         ids=[sw.build.variant_name(*v) for v in sw.build.VARIANTS])
+def test_library_exports(variant):
+    vn = sw.build.variant_name(*variant)
+    lib = ctypes.CDLL(sw.build.library_path(*variant))
+    assert hasattr(lib, "forward")
+    for name in declared():
+        if re.search(r"_[23]d_(cd|vd)_f(32|64)$", name):
+            assert hasattr(lib, name) == name.endswith(vn), name
+        else:
+            assert hasattr(lib, name), name
+    lib.simwave_hip_variant.restype = ctypes.c_char_p
+    assert lib.simwave_hip_variant().decode() == vn
+
+
+def test_fixed_div_helper():
+    lib = ctypes.CDLL(sw.build.library_path(3, "cd", "f32"))
+    fn = lib.simwave_hip_fixed_div_3d_cd_f32
+    fn.argtypes = [ctypes.c_float] + [ctypes.POINTER(ctypes.c_float)] * 3
+    fn.restype = ctypes.c_int
+    y, lo, hi = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    assert fn(156.25, ctypes.byref(y), ctypes.byref(lo), ctypes.byref(hi)) == 1
+    assert abs(y.value * 156.25 - 1.0) < 1e-6 and lo.value < 1e-20 < 1e20 < hi.value
+    assert fn(-1.0, ctypes.byref(y), ctypes.byref(lo), ctypes.byref(hi)) == 0
+
+
+_FAIL_SCRIPT = r"""
+import ctypes, sys
+import numpy as np
+sys.path.insert(0, %(root)r)
+sys.path.insert(0, %(tests)r)
+import cases
+import simwave_amd as sw
+from simwave_amd.middleware import Middleware
+parts = cases.make_case(dim=2, n=24, tf=0.02)()
+solver = sw.Solver(**parts)
+kw = Middleware.normalize(**solver.forward_arguments())
+kw["space_order"] = %(so)d
+lib = ctypes.CDLL(sw.build.library_path(2, "cd", "f32"))
+argtypes, args = Middleware.ordered(kw)
+lib.forward.argtypes = argtypes
+lib.forward.restype = ctypes.c_double
+rc = lib.forward(*args)
+lib.simwave_hip_last_error.restype = ctypes.c_char_p
+print("RC", rc)
+print("ERR", lib.simwave_hip_last_error().decode())
+"""
+
+
+def _run_failing(so):
+    code = _FAIL_SCRIPT % dict(root=ROOT, tests=os.path.join(ROOT, "tests"), so=so)
+    env = dict(os.environ)
+    env["HIP_VISIBLE_DEVICES"] = "-1"      # no device, also on a GPU host
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         timeout=300, env=env)
+    assert res.returncode == 0, res.stderr[-2000:]
+    out = dict(line.split(" ", 1) for line in res.stdout.splitlines()
+               if line.startswith(("RC", "ERR")))
+    return float(out["RC"]), out.get("ERR", "")
+
+
+def test_forward_fails_loudly_without_device():
+    rc, err = _run_failing(4)
+    assert rc < 0
+    assert "device" in err.lower()
+
+
+def test_forward_rejects_bad_space_order():
+    """Validation comes first: an odd order is rejected with its own message
+    whether or not a device is present."""
+    rc, err = _run_failing(3)
+    assert rc < 0
+    assert "space_order 3" in err

@@ -68,6 +68,18 @@ CASES = {
                         src=[(0.0, 100.0, 120.0), (8.0, 5.0, 200.0)], multi=True,
                         tf=0.06),
     "3d_rect": dict(dim=3, shape=(30, 22, 40), space_order=6, tf=0.06),
+    # several y tiles, interior widths that are not whole vectors
+    "3d_wide_so8": dict(dim=3, shape=(26, 30, 301), space_order=8, damping=30.0,
+                        tf=0.05),
+    "3d_wide_f64_so8": dict(dim=3, shape=(22, 27, 263), space_order=8, damping=30.0,
+                            dtype=F64, tf=0.04),
+    "3d_wide_bc": dict(dim=3, shape=(25, 29, 275), space_order=8, damping=0.0,
+                       bc=("null_neumann", "null_dirichlet", "null_neumann",
+                           "null_neumann", "null_neumann", "null_neumann"),
+                       src=[(0.0, 100.0, 2700.0), (60.0, 140.0, 1300.0)], multi=True,
+                       tf=0.05),
+    "3d_wide_so6": dict(dim=3, shape=(24, 26, 270), space_order=6, damping=30.0,
+                        tf=0.05),
 }
 
 
@@ -104,14 +116,36 @@ def test_parity_generic_path(gpu, name):
     _compare(name, CASES[name], generic=True)
 
 
-@pytest.mark.parametrize("rpt", ["2", "3", "4"])
-@pytest.mark.parametrize("name", ["3d_cd_so2", "3d_cd_so8", "3d_cd_so20",
-                                  "3d_bc_mixed", "3d_src_halo", "3d_rect"])
-def test_parity_tiled_rows(gpu, name, rpt):
-    """Every rows-per-thread variant of the tiled 3D kernel."""
-    print("case", name, "rpt", rpt, flush=True)
-    os.environ["SIMWAVE_HIP_RPT"] = rpt
+# Tile configurations of the 3D tiled kernel (SIMWAVE_HIP_TILE): radius 4
+# carries 0..11, other radii 0, 4 and 8; SIMWAVE_HIP_ORDER picks the tile
+# order.
+TILED_R4 = ["3d_cd_so8", "3d_wide_so8", "3d_wide_f64_so8", "3d_wide_bc"]
+TILED_OTHER = ["3d_cd_so2", "3d_cd_so20", "3d_bc_mixed", "3d_src_halo", "3d_rect",
+               "3d_wide_so6"]
+
+
+def _tiled(name, tile, order):
+    print("case", name, "tile", tile, "order", order, flush=True)
+    os.environ["SIMWAVE_HIP_TILE"] = str(tile)
+    os.environ["SIMWAVE_HIP_ORDER"] = str(order)
     try:
         _compare(name, CASES[name], generic=False)
     finally:
-        os.environ.pop("SIMWAVE_HIP_RPT", None)
+        os.environ.pop("SIMWAVE_HIP_TILE", None)
+        os.environ.pop("SIMWAVE_HIP_ORDER", None)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("tile", list(range(12)))
+@pytest.mark.parametrize("name", TILED_R4)
+def test_parity_tiled_configs_r4(gpu, name, tile, order):
+    """Every tile configuration of the radius-4 tiled 3D kernel."""
+    _tiled(name, tile, order)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("tile", [0, 4, 8])
+@pytest.mark.parametrize("name", TILED_OTHER)
+def test_parity_tiled_configs(gpu, name, tile, order):
+    """The tile configurations the other radii carry."""
+    _tiled(name, tile, order)
