@@ -21,6 +21,7 @@ absent, the C restatement in oracle/, on a bounded number of time steps of
 the same grid.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py --copy-calibration   # plain device copy of known size
 """
 import argparse
 import ctypes
@@ -158,6 +159,21 @@ def read_traffic(kernel, workload):
         return json.load(f).get("bytes_per_launch")
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain streaming kernel of known traffic: torch's elementwise
+    y = -x over `nbytes` (reads nbytes, writes nbytes), `reps` times.  It
+    scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel": "elementwise neg (float32)", "bytes_read": nbytes,
+                      "bytes_written": nbytes, "dispatches": reps}), flush=True)
+
+
 # --------------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
@@ -171,8 +187,13 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--time-steps", type=int, default=None,
                     help="truncate each shot to this many time steps (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 1 GiB (counter scale)")
     args = ap.parse_args()
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shots shard across GPUs with no exchange; run one process "
@@ -203,11 +224,14 @@ def main():
         dev.reset()
         dev.run()
     torch.cuda.synchronize()
+    shot_ms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
+        ts = time.perf_counter()
         dev.reset()
-        dev.run()
-        log("shot %d done" % i)
+        dev.run()            # returns after the shot's stream has drained
+        shot_ms.append((time.perf_counter() - ts) * 1e3)
+        log("shot %d done (%.1f ms)" % (i, shot_ms[-1]))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1e3
@@ -225,7 +249,13 @@ def main():
         bpp = (20 if dtype == np.float32 else 40) + (
             (4 if dtype == np.float32 else 8) if kw.get("density_model") is not None else 0)
         achieved = pts * bpp / avg / 1e9
-        kname = "k_step%dd" % kw["velocity_model"].ndim
+        ndim = kw["velocity_model"].ndim
+        if not fused:
+            kname = "k_stencil_generic"
+        elif ndim == 3 and tm[7] == 1.0:
+            kname = "k_step3d_vec"
+        else:
+            kname = "k_step%dd" % ndim
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": read_traffic(kname, args.workload),
@@ -259,6 +289,9 @@ def main():
                    "space_order": kw["space_order"],
                    "receivers": kw["num_receivers"], "parallelism": "none",
                    "kernel_path": "fused" if fused else "generic"},
+        "shot_latency_ms": {"p50": round(float(np.percentile(shot_ms, 50)), 2),
+                            "p99": round(float(np.percentile(shot_ms, 99)), 2),
+                            "shots": len(shot_ms), "load": "one shot at a time"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

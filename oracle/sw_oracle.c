@@ -24,10 +24,14 @@
  * `1.0/(v*v)`, `1.0 +/- ...`, `2.0/den * u`) so that a build with the same
  * flags (-O3 -std=c99, no contraction) is bit-identical to the reference.
  *
- * Parity pinning: tests/test_oracle_vs_reference.py compares every variant
- * bit-for-bit against the reference C compiled from its own sources
- * (oracle/Makefile -> oracle/_ref/), and tests/test_golden.py against the
- * golden wavefields the reference's tests ship.
+ * Parity pinning (tests/test_oracle.py): bit identity with the reference C
+ * compiled from its own sources (oracle/Makefile -> oracle/_ref/), both as
+ * recorded digests (tests/golden/ref_outputs.json, all eight variants'
+ * families incl. the 3D golden grid) and live where oracle/_ref is built;
+ * the golden 2D wavefields the reference's tests ship (atol 1e-5, the
+ * reference's own bar); the reference's invariants (saving stride, OpenMP
+ * vs sequential, unit density) and the analytic 2D Hankel response.  No
+ * function of this file is left unpinned.
  */
 #include <stddef.h>
 #include <sys/time.h>

@@ -741,41 +741,109 @@ void launch_bc_phases(const Params &P, real_t *un, hipStream_t st)
 }
 
 // Runs steps [a.n0, a.n1].  `slot_ptr(i)` returns the device buffer backing
-// logical slot i; `before_step`/`after_step` manage residency (host entry)
-// and `swap` exchanges two logical slots.
-template <typename SlotPtr, typename Before, typename After, typename Swap>
+// logical slot i; `before_step`/`after_step` manage residency (host entry;
+// `uploads(idx)` tells whether before_step(idx) writes a device buffer) and
+// `swap` exchanges two logical slots.
+//
+// The receiver gather of step n only reads u^n (slot cur(n)), which no step
+// before n+2 writes in the reference's slot cycle, so it runs on a second
+// stream concurrently with the following kernels.  Ordering: it starts after
+// everything enqueued before step n on `st` (u^n complete); `st` waits for
+// it before any later step whose output slot is cur(n), before step n+2 in
+// any case, before uploads into recycled buffers and before slot exchanges.
+//
+// With kFlagTimeSteps every step kernel is bracketed by its own event pair
+// on `st` and the durations are summed after the loop, so the measurement
+// adds no synchronisation between steps.
+template <typename SlotPtr, typename Before, typename After, typename Swap,
+          typename Uploads>
 void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &plan,
                real_t *rec_dev, size_t rec_row0, unsigned flags, hipStream_t st,
-               SlotPtr slot_ptr, Before before_step, After after_step, Swap swap)
+               SlotPtr slot_ptr, Before before_step, After after_step, Swap swap,
+               Uploads uploads)
 {
     const Params &P = pr.P;
     const int r = pr.r;
     StepFn fused = RTable::fused[r];
     GenFn generic = RTable::generic[r];
     const bool timed = flags & kFlagTimeSteps;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double step_ms = 0.0;
-    long long step_launches = 0;
+    const size_t nsteps = plan.size();
+    struct Events {
+        std::vector<hipEvent_t> t;
+        hipEvent_t cur = nullptr, rec[3] = {nullptr, nullptr, nullptr};
+        hipStream_t rs = nullptr;
+        ~Events()
+        {
+            for (hipEvent_t e : t)
+                if (e)
+                    (void)hipEventDestroy(e);
+            for (hipEvent_t e : {cur, rec[0], rec[1], rec[2]})
+                if (e)
+                    (void)hipEventDestroy(e);
+            if (rs)
+                (void)hipStreamDestroy(rs);
+        }
+    } ev;
     if (timed) {
-        hip_check(hipEventCreate(&ev0), "hipEventCreate");
-        hip_check(hipEventCreate(&ev1), "hipEventCreate");
+        ev.t.assign(2 * nsteps, nullptr);
+        for (hipEvent_t &e : ev.t)
+            hip_check(hipEventCreate(&e), "hipEventCreate");
     }
+    const bool overlap_rec = a.nrec && !env_flag("SIMWAVE_HIP_REC_SERIAL");
+    if (overlap_rec) {
+        hip_check(hipStreamCreateWithFlags(&ev.rs, hipStreamNonBlocking), "hipStreamCreate");
+        for (hipEvent_t *e : {&ev.cur, &ev.rec[0], &ev.rec[1], &ev.rec[2]})
+            hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+    }
+    // waited[j]: st already waits for the receivers of step j
+    std::vector<char> waited(nsteps, 0);
+    auto wait_rec = [&](long long j) {
+        if (!overlap_rec || j < 0 || waited[j])
+            return;
+        for (long long i = j; i >= 0 && !waited[i]; i--)   // events complete in order
+            waited[i] = 1;
+        hip_check(hipStreamWaitEvent(st, ev.rec[j % 3], 0), "hipStreamWaitEvent");
+    };
     const unsigned rec_blocks = (unsigned)((a.nrec + 255) / 256);
     const bool coop_receivers = !env_flag("SIMWAVE_HIP_REC_THREAD");
     const unsigned coop_blocks = (unsigned)((a.nrec + kRecPerBlock - 1) / kRecPerBlock);
-    for (size_t idx = 0; idx < plan.size(); idx++) {
+    auto receivers = [&](size_t n, const real_t *uc, hipStream_t s) {
+        real_t *row = rec_dev + (n - 1 - rec_row0) * a.nrec;
+        if (coop_receivers)
+            hipLaunchKernelGGL((sw::k_receivers_coop<kRecPerBlock, kRecChunk>),
+                               dim3(coop_blocks), dim3(256), 0, s, P, uc, pr.rbox, pr.rw,
+                               (int)a.nrec, row);
+        else
+            hipLaunchKernelGGL(sw::k_receivers, dim3(rec_blocks), dim3(256), 0, s, P, uc,
+                               pr.rbox, pr.rw, (int)a.nrec, row);
+    };
+    for (size_t idx = 0; idx < nsteps; idx++) {
         const size_t n = a.n0 + idx;
         const StepSlots &s = plan[idx];
+        const long long ii = (long long)idx;
+        // the receivers of steps <= idx-2 are done before step idx; those of
+        // idx-1 too where step idx overwrites their slot or buffers are
+        // recycled by an upload
+        wait_rec(ii - 2);
+        if (idx > 0 && (s.next == plan[idx - 1].cur || uploads(idx)))
+            wait_rec(ii - 1);
         before_step(idx);
         const real_t *up = slot_ptr(s.prev);
         const real_t *uc = slot_ptr(s.cur);
         real_t *un = slot_ptr(s.next);
+        if (overlap_rec) {
+            hip_check(hipEventRecord(ev.cur, st), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(ev.rs, ev.cur, 0), "hipStreamWaitEvent");
+            receivers(n, uc, ev.rs);
+            hip_check(hipEventRecord(ev.rec[idx % 3], ev.rs), "hipEventRecord");
+        }
+        hipEvent_t t0 = timed ? ev.t[2 * idx] : nullptr, t1 = timed ? ev.t[2 * idx + 1] : nullptr;
         if (pr.fused) {
             if (pr.halo_sources)
                 hipLaunchKernelGGL(sw::k_inject, dim3(1), dim3(256), 0, st, P, un,
                                    (int)n, 1);
             if (timed)
-                hip_check(hipEventRecord(ev0, st), "hipEventRecord");
+                hip_check(hipEventRecord(t0, st), "hipEventRecord");
 #if SW_DIM == 3
             if (pr.tiled)
                 pr.tiled_launch(pr, uc, up, un, (int)n, st);
@@ -783,45 +851,42 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
 #endif
                 fused(P, uc, up, un, (int)n, st);
             if (timed)
-                hip_check(hipEventRecord(ev1, st), "hipEventRecord");
+                hip_check(hipEventRecord(t1, st), "hipEventRecord");
         } else {
             if (timed)
-                hip_check(hipEventRecord(ev0, st), "hipEventRecord");
+                hip_check(hipEventRecord(t0, st), "hipEventRecord");
             generic(P, uc, up, un, st);
             if (timed)
-                hip_check(hipEventRecord(ev1, st), "hipEventRecord");
+                hip_check(hipEventRecord(t1, st), "hipEventRecord");
             if (P.nsrc)
                 hipLaunchKernelGGL(sw::k_inject, dim3(1), dim3(256), 0, st, P, un,
                                    (int)n, 0);
             launch_bc_phases(P, un, st);
         }
-        if (a.nrec) {
-            real_t *row = rec_dev + (n - 1 - rec_row0) * a.nrec;
-            if (coop_receivers)
-                hipLaunchKernelGGL((sw::k_receivers_coop<kRecPerBlock, kRecChunk>),
-                                   dim3(coop_blocks), dim3(256), 0, st, P, uc, pr.rbox,
-                                   pr.rw, (int)a.nrec, row);
-            else
-                hipLaunchKernelGGL(sw::k_receivers, dim3(rec_blocks), dim3(256), 0, st, P,
-                                   uc, pr.rbox, pr.rw, (int)a.nrec, row);
-        }
+        if (a.nrec && !overlap_rec)
+            receivers(n, uc, st);
         hip_check(hipGetLastError(), "step launch");
-        if (timed) {
-            float ms = 0.f;
-            hip_check(hipEventSynchronize(ev1), "hipEventSynchronize");
-            hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
-            step_ms += ms;
-            step_launches++;
-        }
-        if (s.swap_after)
+        if (s.swap_after) {
+            wait_rec(ii);
             swap(s.swap_a, s.swap_b);
+        }
         after_step(idx);
     }
+    if (nsteps)
+        wait_rec((long long)nsteps - 1);
+    if (overlap_rec)
+        hip_check(hipStreamSynchronize(ev.rs), "receiver stream");
     if (timed) {
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
+        hip_check(hipStreamSynchronize(st), "timed steps");
+        double step_ms = 0.0;
+        for (size_t i = 0; i < nsteps; i++) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, ev.t[2 * i], ev.t[2 * i + 1]),
+                      "hipEventElapsedTime");
+            step_ms += ms;
+        }
         g_timings[4] = step_ms * 1e-3;
-        g_timings[5] = (double)step_launches;
+        g_timings[5] = (double)nsteps;
     }
 }
 
@@ -920,7 +985,9 @@ double run_host(const Args &a)
         auto slot_ptr = [&](size_t sl) { return phys[sl]; };
         auto swap = [&](size_t x, size_t y) { std::swap(phys[x], phys[y]); };
 
-        time_loop(a, pr, plan, rec_dev, a.n0 - 1, 0u, st, slot_ptr, before, after, swap);
+        auto uploads = [&](size_t idx) { return !up_at[idx].empty(); };
+        time_loop(a, pr, plan, rec_dev, a.n0 - 1, 0u, st, slot_ptr, before, after, swap,
+                  uploads);
         hip_check(hipStreamSynchronize(st), "time loop");
         const double tl1 = now_s();
         t_loop = tl1 - tl0;
@@ -931,6 +998,9 @@ double run_host(const Args &a)
         hip_check(hipStreamSynchronize(st), "download");
         t_down = now_s() - tl1;
         g_timings[6] = pr.fused ? 1.0 : 0.0;
+#if SW_DIM == 3
+        g_timings[7] = pr.tiled ? 1.0 : 0.0;
+#endif
     }
     const double total = now_s() - t0;
     g_timings[0] = total;
@@ -973,10 +1043,14 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
         hipLaunchKernelGGL(sw::k_swap, dim3(blocks), dim3(256), 0, st,
                            a.u + x * npts, a.u + y * npts, (long long)npts);
     };
-    time_loop(a, pr, plan, a.receivers, 0, flags, st, slot_ptr, nop, nop, swap);
+    auto no_uploads = [](size_t) { return false; };
+    time_loop(a, pr, plan, a.receivers, 0, flags, st, slot_ptr, nop, nop, swap, no_uploads);
     hip_check(hipStreamSynchronize(st), "time loop");
     g_timings[2] = now_s() - tl0;
     g_timings[6] = pr.fused ? 1.0 : 0.0;
+#if SW_DIM == 3
+    g_timings[7] = pr.tiled ? 1.0 : 0.0;
+#endif
     const double total = now_s() - t0;
     g_timings[0] = total;
     return total;

@@ -242,7 +242,8 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     const rsrc_t r_e = make_rsrc(A.e + off0 / esz, rem);
     const rsrc_t r_un = make_rsrc(A.un + off0 / esz, rem);
 
-    // This lane's points: rows j0 + slot + SLOTS*q, columns k .. k+V-1.
+    // This lane's points: rows j0 + slot*RPT + q (adjacent rows share their
+    // x neighbours), columns k .. k+V-1.
     // Lanes past the row end read the row start (values unused); a lane
     // straddling the row end reads at most V-1 elements of the next row.
     const int k = k0 + lc * V;
@@ -251,7 +252,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     bool act[RPT];      // row computed
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
-        const int j = j0 + slot + SLOTS * q;
+        const int j = j0 + slot * RPT + q;
         act[q] = j < nx - R;
         const int jl = j < nx ? j : nx - 1;
         cb[q] = ((unsigned)jl * (unsigned)ny + (unsigned)kl) * esz;
@@ -357,7 +358,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
             for (int m = 0; m < 2 * R; m++)
                 qz[q][m] = qz[q][m + 1];
             qz[q][2 * R] = nc[q];
-            *(vreal *)&lb[(R + slot + SLOTS * q) * LS + PADL + lc * V] = qz[q][R];
+            *(vreal *)&lb[(R + slot * RPT + q) * LS + PADL + lc * V] = qz[q][R];
         }
 #pragma unroll
         for (int h = 0; h < HX; h++)
@@ -376,9 +377,20 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
         const bool special = (smask != 0) | bc_here;
         const unsigned so = (unsigned)(i - zb) * PB;
         vreal out[RPT];
+        // x neighbours of the lane's RPT adjacent rows: rows -R .. RPT-1+R
+        // around its first row, the own rows from registers
+        // (one row per lane: read inside the stencil loop instead, which
+        // keeps fewer vectors live)
+        const real_t *lrow0 = lb + (R + slot * RPT) * LS + lc * V + PADL;
+        vreal X[RPT + 2 * R];
+        if constexpr (RPT > 1) {
+#pragma unroll
+            for (int d = -R; d < RPT + R; d++)
+                X[d + R] = (d >= 0 && d < RPT) ? qz[d][R] : *(const vreal *)&lrow0[d * LS];
+        }
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
-            const int row = R + slot + SLOTS * q;
+            const int row = R + slot * RPT + q;
             const real_t *lrow = lb + row * LS + lc * V;   // y window start
             // y window: points k-PADL .. k+V+PADL-1
             real_t win[V + 2 * PADL];
@@ -405,8 +417,14 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
             }
 #pragma unroll
             for (int m = 1; m <= R; m++) {
-                const vreal xp = *(const vreal *)&lrow[m * LS + PADL];
-                const vreal xm = *(const vreal *)&lrow[-m * LS + PADL];
+                vreal xp, xm;
+                if constexpr (RPT > 1) {
+                    xp = X[q + m + R];
+                    xm = X[q - m + R];
+                } else {
+                    xp = *(const vreal *)&lrow0[m * LS];
+                    xm = *(const vreal *)&lrow0[-m * LS];
+                }
 #pragma unroll
                 for (int e = 0; e < V; e++) {
                     acc_y[e] += A.c2[m] * (win[PADL + e + m] + win[PADL + e - m]);
@@ -449,7 +467,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
             }
             if (special && act[q]) {
                 // tiles touching a source box or a boundary face
-                const int j = j0 + slot + SLOTS * q;
+                const int j = j0 + slot * RPT + q;
 #pragma unroll
                 for (int e = 0; e < V; e++) {
                     const int kk = k + e;
