@@ -91,12 +91,144 @@ def readme_2d(dtype=np.float32):
                 wavelet=sw.RickerWavelet(10.0, tm))
 
 
+def serving_requests(count, seed=0, n2=(256, 1024), n3=(64, 160), steps=(100, 300)):
+    """Seeded stream of shot requests for the serving workload: 2D or 3D
+    (50/50), 2D grids n2[0]^2..n2[1]^2 and 3D grids n3[0]^3..n3[1]^3 at
+    10 m, space order uniform in {2,4,8,12,16,20}, constant or variable
+    density 50/50 (Gardner), 3-8 random flat layers of 1500-5000 m/s, a
+    5-40 Hz Ricker at the shallow centre, 64-4096 surface receivers, 20-cell
+    damping, float32, tf set for steps[0]..steps[1] time steps."""
+    import simwave_amd as sw
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        dim = 2 if rng.random() < 0.5 else 3
+        n = int(rng.integers(*(n2 if dim == 2 else n3), endpoint=True))
+        so = int(rng.choice([2, 4, 8, 12, 16, 20]))
+        dens = bool(rng.random() < 0.5)
+        layers = np.sort(rng.uniform(1500.0, 5000.0, int(rng.integers(3, 8, endpoint=True))))
+        f0 = float(rng.uniform(5.0, 40.0))
+        nrec = int(rng.integers(64, 4096, endpoint=True))
+        nsteps = int(rng.integers(*steps, endpoint=True))
+        out.append(dict(dim=dim, n=n, so=so, dens=dens, layers=layers, f0=f0,
+                        nrec=nrec, nsteps=nsteps))
+
+    def build(req):
+        dim, n, h = req["dim"], req["n"], 10.0
+        shape = (n,) * dim
+        nl = len(req["layers"])
+        vel = np.repeat(req["layers"], -(-n // nl))[:n].astype(np.float32)
+        vel = np.broadcast_to(vel.reshape((n,) + (1,) * (dim - 1)), shape).copy()
+        den = (310.0 * vel.astype(np.float64) ** 0.25).astype(np.float32) if req["dens"] else None
+        ext = (n - 1) * h
+        sm = sw.SpaceModel(bounding_box=(0, ext) * dim, grid_spacing=(h,) * dim,
+                           velocity_model=vel, density_model=den,
+                           space_order=req["so"], dtype=np.float32)
+        sm.config_boundary(damping_length=20 * h,
+                           boundary_condition=("null_neumann",) + ("null_dirichlet",) * (2 * dim - 1))
+        dt = sw.TimeModel(space_model=sm, tf=1.0).dt
+        tm = sw.TimeModel(space_model=sm, tf=float(dt) * (req["nsteps"] - 1))
+        src = sw.Source(sm, coordinates=[(2 * h,) + (ext / 2,) * (dim - 1)])
+        if dim == 2:
+            xs = np.linspace(0.0, ext, req["nrec"])
+            rc = [(h, x) for x in xs]
+        else:
+            m = int(np.ceil(np.sqrt(req["nrec"])))
+            g = np.linspace(0.0, ext, m)
+            rc = [(h, x, y) for x in g for y in g][:req["nrec"]]
+        rec = sw.Receiver(sm, coordinates=rc)
+        return sw.Solver(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                         wavelet=sw.RickerWavelet(req["f0"], tm))
+
+    return out, build
+
+
+def run_serving(args):
+    """Serving workload: one JSON line with the aggregate Gpts/s of the
+    server under Poisson arrivals, p50/p99 latency and the plain path (the
+    same shots one at a time) beside it."""
+    import torch
+    from simwave_amd.device import DeviceForward
+    from simwave_amd.serving import ShotServer
+
+    reqs, build = serving_requests(args.serve_requests, seed=args.seed)
+    t0 = time.time()
+    solvers = [build(r) for r in reqs]
+    log("serving: %d requests built in %.1fs" % (len(solvers), time.time() - t0))
+    warm = [build(r) for r in serving_requests(2, seed=args.seed + 1000,
+                                               n2=(128, 128), n3=(32, 32),
+                                               steps=(20, 20))[0]]
+    log("serving: warm-up (%d small shots)" % len(warm))
+    with ShotServer(hbm_budget=args.hbm_budget * 1e9, max_active=args.max_active,
+                    chunk_steps=args.chunk_steps) as srv:
+        for f in [srv.submit(s) for s in warm]:
+            f.result(timeout=120)
+    log("serving: warm-up done")
+    rng = np.random.default_rng(args.seed + 1)
+    gaps = rng.exponential(1.0 / args.rate, size=len(solvers))
+    torch.cuda.synchronize()
+    with ShotServer(hbm_budget=args.hbm_budget * 1e9, max_active=args.max_active,
+                    chunk_steps=args.chunk_steps) as srv:
+        start = time.perf_counter()
+        futs = []
+        due = start
+        for s, g in zip(solvers, gaps):
+            due += g
+            delay = due - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(srv.submit(s, arrival=due))
+        for i, f in enumerate(futs):
+            f.result(timeout=300)
+            log("serving: request %d/%d done" % (i + 1, len(futs)))
+    summ = srv.stats.summary()
+    log("serving: %s" % summ)
+    # plain path: the same shots one at a time, each alone on the device
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    upd = 0
+    for i, s in enumerate(solvers):
+        dev = DeviceForward(s)
+        dev.run()
+        dev.results()
+        upd += dev.updates_per_shot
+        del dev
+        if (i + 1) % 8 == 0:
+            log("plain path: %d/%d shots" % (i + 1, len(solvers)))
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - tp
+    plain = upd / plain_s / 1e9
+    log("plain path: %d shots in %.2fs, %.2f Gpts/s" % (len(solvers), plain_s, plain))
+    out = {
+        "metric": METRIC, "value": round(summ["gpts"], 3), "unit": "Gpts/s",
+        "n_gpus": 1, "steps": len(solvers), "warmup": len(warm),
+        "ms_per_step": round(summ["makespan_s"] / len(solvers) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving", "requests": len(solvers), "seed": args.seed,
+                   "arrival_rate_hz": args.rate, "hbm_budget_gb": args.hbm_budget,
+                   "max_active": args.max_active, "chunk_steps": args.chunk_steps,
+                   "description": serving_requests.__doc__.split("\n")[0]},
+        "shot_latency_ms": {"p50": round(summ["latency_p50_s"] * 1e3, 2),
+                            "p99": round(summ["latency_p99_s"] * 1e3, 2),
+                            "load": "Poisson arrivals at %g/s" % args.rate},
+        "serving": {"updates": summ["updates"], "makespan_s": round(summ["makespan_s"], 3),
+                    "queue_p50_ms": round(summ["queue_p50_s"] * 1e3, 2),
+                    "max_active_seen": summ["max_active"],
+                    "plain_path_gpts": round(plain, 3),
+                    "speedup_vs_plain": round(summ["gpts"] / plain, 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 WORKLOADS = {
     "3d_cd_so8_384": (headline_3d,
                       "3D constant density so8 384^3 (472^3 extended), "
                       "12.5 m, 40-pt cubic damping, tf 1.5 s, 128x128 receivers"),
     "2d_cd_so4_readme": (readme_2d,
                          "2D constant density so4 512^2 (README example)"),
+    "serving": (None, "mixed shot stream through the continuous-batching server"),
 }
 
 
@@ -187,6 +319,12 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--time-steps", type=int, default=None,
                     help="truncate each shot to this many time steps (profiling)")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=400.0, help="serving arrivals per second")
+    ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving budget, GB")
+    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--chunk-steps", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (counter scale)")
     args = ap.parse_args()
@@ -207,6 +345,9 @@ def main():
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "serving":
+        run_serving(args)
+        return
     dtype = np.float32 if args.dtype == "f32" else np.float64
     builder, desc = WORKLOADS[args.workload]
     t0 = time.time()

@@ -13,6 +13,8 @@
  *   - simwave_hip_forward_<variant>: the same function under a unique name;
  *   - simwave_hip_forward_device_<variant>: the same contract with device
  *     pointers for the arrays that live on the GPU across calls;
+ *   - simwave_hip_plan_{create,run,destroy}_<variant>: the device entry
+ *     split into chunks for serving loops;
  *   - simwave_hip_fixed_div_<variant>, simwave_hip_last_error,
  *     simwave_hip_last_timings, simwave_hip_variant.
  *
@@ -78,6 +80,34 @@ double simwave_hip_forward_device_2d_cd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_2d_cd_f32; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_2d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_2d_cd_f32(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -124,6 +154,34 @@ double simwave_hip_forward_device_2d_cd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_2d_cd_f64; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_2d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_2d_cd_f64(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -170,6 +228,34 @@ double simwave_hip_forward_device_2d_vd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_2d_vd_f32; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_2d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_2d_vd_f32(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -216,6 +302,34 @@ double simwave_hip_forward_device_2d_vd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_2d_vd_f64; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_2d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_2d_vd_f64(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -262,6 +376,34 @@ double simwave_hip_forward_device_3d_cd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_3d_cd_f32; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_3d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_3d_cd_f32(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -308,6 +450,34 @@ double simwave_hip_forward_device_3d_cd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_3d_cd_f64; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_3d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_3d_cd_f64(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -354,6 +524,34 @@ double simwave_hip_forward_device_3d_vd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_3d_vd_f32; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_3d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_3d_vd_f32(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -400,6 +598,34 @@ double simwave_hip_forward_device_3d_vd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Persistent plan (no counterpart in the reference, whose forward runs a
+ * whole step range per call): the device entry split into create / run in
+ * chunks / destroy, for callers that interleave shots (the serving loop) or
+ * re-run step ranges (checkpointed recomputation).  Arguments as
+ * simwave_hip_forward_device_3d_vd_f64; its host arrays are read only here.
+ * Returns NULL on failure. */
+void *simwave_hip_plan_create_3d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Enqueues the next `steps` steps of the plan on `stream`, ordered after the
+ * plan's earlier work on any stream; does not wait for the device.  Returns
+ * the steps left, or -1. */
+long long simwave_hip_plan_run_3d_vd_f64(void *plan, size_t steps, void *stream);
+/* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
+int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe

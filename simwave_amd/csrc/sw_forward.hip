@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <memory>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -257,6 +258,7 @@ struct Prepared {
     void (*tiled_launch)(const struct Prepared &, const real_t *, const real_t *,
                          real_t *, int, hipStream_t) = nullptr;
     const Params *Pd = nullptr;   // device copy of P for the tiled kernel
+    const real_t *qc = nullptr;   // (dt2/s)/den per point for the tiled kernel
     bool fastdiv = false;
     sw::FixedDiv dvz{}, dvx{}, dvy{};
     sw::Tiling tiling{};
@@ -485,7 +487,7 @@ bool make_fixed_div(real_t b, sw::FixedDiv &d)
 // thread RPT, wavefronts per block NW).  Radius 4 (space order 8) carries
 // every configuration; other radii carry 0, 4 and 8.  SIMWAVE_HIP_TILE picks
 // one; otherwise default_tile(R).
-constexpr int kTileShapes = 12;
+constexpr int kTileShapes = 14;
 constexpr int default_tile(int R) { return R == 4 ? 10 : R < 4 ? 0 : 4; }
 
 template <int R, int WR, int RPT, int NW>
@@ -496,7 +498,7 @@ void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t
     sw::StepArgs A{};
     A.uc = uc;
     A.up = up;
-    A.q = P.q;
+    A.qc = pr.qc;
     A.e = P.e;
     A.un = un;
     A.fastdiv = pr.fastdiv ? 1 : 0;
@@ -554,6 +556,8 @@ TiledKernel tiled_entry(int v)
         case 9: return tiled_config<R, 8, 1, 2>();
         case 10: return tiled_config<R, 4, 2, 1>();
         case 11: return tiled_config<R, 4, 1, 1>();
+        case 12: return tiled_config<R, 4, 1, 8>();
+        case 13: return tiled_config<R, 2, 1, 8>();
         default: return tiled_config<R, 1, 1, 4>();
         }
     } else {
@@ -636,6 +640,14 @@ void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
     hip_check(hipMemcpyAsync(Pd, &P, sizeof(Params), hipMemcpyHostToDevice, st),
               "copy params");
     pr.Pd = Pd;
+    {
+        real_t *qc = arena.alloc<real_t>(pr.npts);
+        const int nb = (int)std::min<long long>((pr.npts + 255) / 256, 8192);
+        hipLaunchKernelGGL(sw::k_scaled_q, dim3(nb), dim3(256), 0, st, P.q, P.e, qc,
+                           pr.npts);
+        hip_check(hipGetLastError(), "k_scaled_q launch");
+        pr.qc = qc;
+    }
     pr.fastdiv = !env_flag("SIMWAVE_HIP_IEEE_DIV") && make_fixed_div(P.dz2, pr.dvz) &&
                  make_fixed_div(P.dx2, pr.dvx) && make_fixed_div(P.dy2, pr.dvy);
     pr.tiled_launch = K.launch;
@@ -740,6 +752,31 @@ void launch_bc_phases(const Params &P, real_t *un, hipStream_t st)
     }
 }
 
+// Second stream and events for the receiver gather (see time_loop).  A
+// plan owns one across its runs; a one-shot call makes its own.
+struct RecStream {
+    hipStream_t rs = nullptr;
+    hipEvent_t cur = nullptr, rec[3] = {nullptr, nullptr, nullptr};
+    void create()
+    {
+        if (rs)
+            return;
+        hip_check(hipStreamCreateWithFlags(&rs, hipStreamNonBlocking), "hipStreamCreate");
+        for (hipEvent_t *e : {&cur, &rec[0], &rec[1], &rec[2]})
+            hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+    }
+    ~RecStream()
+    {
+        if (rs)
+            (void)hipStreamSynchronize(rs);
+        for (hipEvent_t e : {cur, rec[0], rec[1], rec[2]})
+            if (e)
+                (void)hipEventDestroy(e);
+        if (rs)
+            (void)hipStreamDestroy(rs);
+    }
+};
+
 // Runs steps [a.n0, a.n1].  `slot_ptr(i)` returns the device buffer backing
 // logical slot i; `before_step`/`after_step` manage residency (host entry;
 // `uploads(idx)` tells whether before_step(idx) writes a device buffer) and
@@ -758,44 +795,37 @@ void launch_bc_phases(const Params &P, real_t *un, hipStream_t st)
 template <typename SlotPtr, typename Before, typename After, typename Swap,
           typename Uploads>
 void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &plan,
-               real_t *rec_dev, size_t rec_row0, unsigned flags, hipStream_t st,
-               SlotPtr slot_ptr, Before before_step, After after_step, Swap swap,
-               Uploads uploads)
+               size_t i0, size_t i1, real_t *rec_dev, size_t rec_row0, unsigned flags,
+               hipStream_t st, SlotPtr slot_ptr, Before before_step, After after_step,
+               Swap swap, Uploads uploads, RecStream *rec_stream = nullptr)
 {
     const Params &P = pr.P;
     const int r = pr.r;
     StepFn fused = RTable::fused[r];
     GenFn generic = RTable::generic[r];
     const bool timed = flags & kFlagTimeSteps;
-    const size_t nsteps = plan.size();
-    struct Events {
+    // plan entries i0 .. i1-1 (step a.n0 + idx for entry idx)
+    const size_t nsteps = i1 - i0;
+    struct Timing {
         std::vector<hipEvent_t> t;
-        hipEvent_t cur = nullptr, rec[3] = {nullptr, nullptr, nullptr};
-        hipStream_t rs = nullptr;
-        ~Events()
+        ~Timing()
         {
             for (hipEvent_t e : t)
                 if (e)
                     (void)hipEventDestroy(e);
-            for (hipEvent_t e : {cur, rec[0], rec[1], rec[2]})
-                if (e)
-                    (void)hipEventDestroy(e);
-            if (rs)
-                (void)hipStreamDestroy(rs);
         }
-    } ev;
+    } tev;
+    RecStream local;
+    RecStream &ev = rec_stream ? *rec_stream : local;
     if (timed) {
-        ev.t.assign(2 * nsteps, nullptr);
-        for (hipEvent_t &e : ev.t)
+        tev.t.assign(2 * nsteps, nullptr);
+        for (hipEvent_t &e : tev.t)
             hip_check(hipEventCreate(&e), "hipEventCreate");
     }
     const bool overlap_rec = a.nrec && !env_flag("SIMWAVE_HIP_REC_SERIAL");
-    if (overlap_rec) {
-        hip_check(hipStreamCreateWithFlags(&ev.rs, hipStreamNonBlocking), "hipStreamCreate");
-        for (hipEvent_t *e : {&ev.cur, &ev.rec[0], &ev.rec[1], &ev.rec[2]})
-            hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
-    }
-    // waited[j]: st already waits for the receivers of step j
+    if (overlap_rec)
+        ev.create();
+    // waited[j]: st already waits for the receivers of entry i0 + j
     std::vector<char> waited(nsteps, 0);
     auto wait_rec = [&](long long j) {
         if (!overlap_rec || j < 0 || waited[j])
@@ -817,15 +847,15 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
             hipLaunchKernelGGL(sw::k_receivers, dim3(rec_blocks), dim3(256), 0, s, P, uc,
                                pr.rbox, pr.rw, (int)a.nrec, row);
     };
-    for (size_t idx = 0; idx < nsteps; idx++) {
+    for (size_t idx = i0; idx < i1; idx++) {
         const size_t n = a.n0 + idx;
         const StepSlots &s = plan[idx];
-        const long long ii = (long long)idx;
+        const long long ii = (long long)(idx - i0);
         // the receivers of steps <= idx-2 are done before step idx; those of
         // idx-1 too where step idx overwrites their slot or buffers are
         // recycled by an upload
         wait_rec(ii - 2);
-        if (idx > 0 && (s.next == plan[idx - 1].cur || uploads(idx)))
+        if (ii > 0 && (s.next == plan[idx - 1].cur || uploads(idx)))
             wait_rec(ii - 1);
         before_step(idx);
         const real_t *up = slot_ptr(s.prev);
@@ -835,9 +865,9 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
             hip_check(hipEventRecord(ev.cur, st), "hipEventRecord");
             hip_check(hipStreamWaitEvent(ev.rs, ev.cur, 0), "hipStreamWaitEvent");
             receivers(n, uc, ev.rs);
-            hip_check(hipEventRecord(ev.rec[idx % 3], ev.rs), "hipEventRecord");
+            hip_check(hipEventRecord(ev.rec[ii % 3], ev.rs), "hipEventRecord");
         }
-        hipEvent_t t0 = timed ? ev.t[2 * idx] : nullptr, t1 = timed ? ev.t[2 * idx + 1] : nullptr;
+        hipEvent_t t0 = timed ? tev.t[2 * ii] : nullptr, t1 = timed ? tev.t[2 * ii + 1] : nullptr;
         if (pr.fused) {
             if (pr.halo_sources)
                 hipLaunchKernelGGL(sw::k_inject, dim3(1), dim3(256), 0, st, P, un,
@@ -872,16 +902,15 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
         }
         after_step(idx);
     }
+    // st depends on every receiver gather of the range before it goes on
     if (nsteps)
         wait_rec((long long)nsteps - 1);
-    if (overlap_rec)
-        hip_check(hipStreamSynchronize(ev.rs), "receiver stream");
     if (timed) {
         hip_check(hipStreamSynchronize(st), "timed steps");
         double step_ms = 0.0;
         for (size_t i = 0; i < nsteps; i++) {
             float ms = 0.f;
-            hip_check(hipEventElapsedTime(&ms, ev.t[2 * i], ev.t[2 * i + 1]),
+            hip_check(hipEventElapsedTime(&ms, tev.t[2 * i], tev.t[2 * i + 1]),
                       "hipEventElapsedTime");
             step_ms += ms;
         }
@@ -986,8 +1015,8 @@ double run_host(const Args &a)
         auto swap = [&](size_t x, size_t y) { std::swap(phys[x], phys[y]); };
 
         auto uploads = [&](size_t idx) { return !up_at[idx].empty(); };
-        time_loop(a, pr, plan, rec_dev, a.n0 - 1, 0u, st, slot_ptr, before, after, swap,
-                  uploads);
+        time_loop(a, pr, plan, 0, plan.size(), rec_dev, a.n0 - 1, 0u, st, slot_ptr, before,
+                  after, swap, uploads);
         hip_check(hipStreamSynchronize(st), "time loop");
         const double tl1 = now_s();
         t_loop = tl1 - tl0;
@@ -1013,18 +1042,38 @@ double run_host(const Args &a)
 // -------------------------------------------------------- device entry
 // u, velocity, density, damp and receivers are device pointers; u holds all
 // num_snapshots slots, receivers wavelet_size rows.  Everything else is host
-// memory.  Runs on `stream` (nullptr: the null stream) and returns after the
-// stream has drained; the return value is wall seconds of the call.
-double run_device(const Args &a, hipStream_t st, unsigned flags)
+// memory, read only while the plan is created.  A plan keeps everything a
+// shot needs on the device and runs its steps in order, in any number of
+// chunks, on any stream.
+struct Plan {
+    Args a;
+    unsigned flags;
+    hipStream_t st;              // stream of creation; the arena frees on it
+    DeviceArena arena;
+    Prepared pr;
+    std::vector<StepSlots> slots;
+    size_t done = 0;             // plan entries enqueued so far
+    hipEvent_t last = nullptr;   // end of the latest enqueued work
+    RecStream rec;
+    Plan(const Args &args, hipStream_t s, unsigned f) : a(args), flags(f), st(s), arena(s) {}
+    ~Plan()
+    {
+        // the arena frees on `st`: order it after the latest run
+        if (last) {
+            (void)hipStreamWaitEvent(st, last, 0);
+            (void)hipEventDestroy(last);
+        }
+    }
+};
+
+Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
 {
-    const double t0 = now_s();
-    for (double &t : g_timings)
-        t = 0.0;
     validate(a);
     keep_pool_memory();
-    std::vector<StepSlots> plan = plan_slots(a);
-    for (size_t idx = 0; idx < plan.size(); idx++) {
-        const StepSlots &s = plan[idx];
+    std::unique_ptr<Plan> p(new Plan(a, st, flags));
+    p->slots = plan_slots(a);
+    for (size_t idx = 0; idx < p->slots.size(); idx++) {
+        const StepSlots &s = p->slots[idx];
         size_t m = std::max({s.prev, s.cur, s.next});
         if (s.swap_after)
             m = std::max({m, s.swap_a, s.swap_b});
@@ -1032,25 +1081,51 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
             fail("slot %zu used by step %zu is outside the %zu snapshots", m,
                  a.n0 + idx, a.num_snapshots);
     }
-    DeviceArena arena(st);
-    Prepared pr = prepare(a, true, flags, arena, st);
-    const size_t npts = (size_t)pr.npts;
-    const double tl0 = now_s();
-    auto slot_ptr = [&](size_t sl) { return a.u + sl * npts; };
+    p->pr = prepare(a, true, flags, p->arena, st);
+    hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(p->last, st), "hipEventRecord");
+    return p.release();
+}
+
+// Enqueues the next `count` steps of the plan on `st` (after all earlier
+// work of the plan, whatever stream it ran on); returns the steps left.
+size_t run_plan(Plan &p, size_t count, hipStream_t st)
+{
+    const size_t i0 = p.done, i1 = std::min(p.slots.size(), i0 + count);
+    hip_check(hipStreamWaitEvent(st, p.last, 0), "hipStreamWaitEvent");
+    const size_t npts = (size_t)p.pr.npts;
+    real_t *u = p.a.u;
+    auto slot_ptr = [&](size_t sl) { return u + sl * npts; };
     auto nop = [](size_t) {};
     auto swap = [&](size_t x, size_t y) {
         const int blocks = (int)std::min<long long>((long long)(npts + 255) / 256, 16384);
-        hipLaunchKernelGGL(sw::k_swap, dim3(blocks), dim3(256), 0, st,
-                           a.u + x * npts, a.u + y * npts, (long long)npts);
+        hipLaunchKernelGGL(sw::k_swap, dim3(blocks), dim3(256), 0, st, u + x * npts,
+                           u + y * npts, (long long)npts);
     };
     auto no_uploads = [](size_t) { return false; };
-    time_loop(a, pr, plan, a.receivers, 0, flags, st, slot_ptr, nop, nop, swap, no_uploads);
+    time_loop(p.a, p.pr, p.slots, i0, i1, p.a.receivers, 0, p.flags, st, slot_ptr, nop,
+              nop, swap, no_uploads, &p.rec);
+    hip_check(hipEventRecord(p.last, st), "hipEventRecord");
+    p.done = i1;
+    g_timings[6] = p.pr.fused ? 1.0 : 0.0;
+#if SW_DIM == 3
+    g_timings[7] = p.pr.tiled ? 1.0 : 0.0;
+#endif
+    return p.slots.size() - p.done;
+}
+
+// One whole shot on `stream`; returns after the stream has drained (wall
+// seconds of the call).
+double run_device(const Args &a, hipStream_t st, unsigned flags)
+{
+    const double t0 = now_s();
+    for (double &t : g_timings)
+        t = 0.0;
+    std::unique_ptr<Plan> p(make_plan(a, st, flags));
+    const double tl0 = now_s();
+    run_plan(*p, p->slots.size(), st);
     hip_check(hipStreamSynchronize(st), "time loop");
     g_timings[2] = now_s() - tl0;
-    g_timings[6] = pr.fused ? 1.0 : 0.0;
-#if SW_DIM == 3
-    g_timings[7] = pr.tiled ? 1.0 : 0.0;
-#endif
     const double total = now_s() - t0;
     g_timings[0] = total;
     return total;
@@ -1177,6 +1252,40 @@ SW_EXPORT double SW_SUFFIX(simwave_hip_forward_device_)(SW_FORWARD_PARAMS, void 
     return guarded([&] {
         return run_device(fill_args(SW_FORWARD_ARGS), (hipStream_t)stream, flags);
     });
+}
+
+SW_EXPORT void *SW_SUFFIX(simwave_hip_plan_create_)(SW_FORWARD_PARAMS, void *stream,
+                                                    unsigned flags)
+{
+    Plan *out = nullptr;
+    const double r = guarded([&] {
+        for (double &t : g_timings)
+            t = 0.0;
+        out = make_plan(fill_args(SW_FORWARD_ARGS), (hipStream_t)stream, flags);
+        return 0.0;
+    });
+    return r < 0 ? nullptr : out;
+}
+
+SW_EXPORT long long SW_SUFFIX(simwave_hip_plan_run_)(void *plan, size_t steps, void *stream)
+{
+    long long left = -1;
+    const double r = guarded([&] {
+        if (!plan)
+            fail("NULL plan");
+        left = (long long)run_plan(*static_cast<Plan *>(plan), steps, (hipStream_t)stream);
+        return 0.0;
+    });
+    return r < 0 ? -1 : left;
+}
+
+SW_EXPORT int SW_SUFFIX(simwave_hip_plan_destroy_)(void *plan)
+{
+    const double r = guarded([&] {
+        delete static_cast<Plan *>(plan);
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
 }
 
 SW_EXPORT const char *simwave_hip_last_error(void)

@@ -200,6 +200,19 @@ __global__ void k_prepare(const real_t *vel, const real_t *damp,
     }
 }
 
+// C = (dt2/s)/den with den = (float)(1.0 + e): the reference's per-point
+// factor `(dtSquared / slowness) / denominator` (cd-3d wave.c:184), which
+// does not depend on the step.  C == q where e == 0.
+__global__ void k_scaled_q(const real_t *__restrict__ q, const real_t *__restrict__ e,
+                           real_t *__restrict__ C, long long npts)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         p < npts; p += (long long)gridDim.x * blockDim.x) {
+        const real_t den = (real_t)(1.0 + (double)e[p]);
+        C[p] = q[p] / den;
+    }
+}
+
 // -------------------------------------------------- boundary predicates
 // For grids whose interior spans at least r+1 points on every axis, the
 // Neumann mirrors only read interior points and only write halo points, so

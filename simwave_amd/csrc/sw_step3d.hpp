@@ -9,9 +9,9 @@
 //   * z neighbours: a per-lane register queue of 2R+1 vectors per row;
 //   * x/y neighbours: the current plane in LDS with an R-wide halo (rounded
 //     up to whole vectors), double buffered so one barrier per plane suffices;
-//   * the next plane's centre values, halos, u_prev, q and (on planes that
-//     hold damping) the damping term are loaded while the current plane is
-//     computed;
+//   * the next plane's centre values, halos, u_prev, C = (dt2/s)/den and
+//     (on planes that hold damping) the damping term are loaded while the
+//     current plane is computed;
 //   * every global access is a buffer instruction on a descriptor rebased at
 //     the tile's first plane.  The z-queue loads carry the plane offset in the
 //     VGPR offset, so the descriptor's range check (which covers the VGPR
@@ -138,14 +138,15 @@ __device__ __forceinline__ void sstore(real_t v, rsrc_t r, unsigned voff, unsign
 }
 
 // Leap-frog update on a plane holding damping: the reference's full
-// expression (wave.c:177-185).  With e == 0 it gives the same bits as
-// leapfrog_plain, so one form serves every point of such a plane.
-__device__ __forceinline__ real_t leapfrog_damped(real_t value, real_t q, real_t e,
+// expression (wave.c:177-185) with its step-independent factor
+// C = (dt2/s)/den precomputed (k_scaled_q).  With e == 0 it gives the same
+// bits as leapfrog_plain, so one form serves every point of such a plane.
+__device__ __forceinline__ real_t leapfrog_damped(real_t value, real_t C, real_t e,
                                                   real_t uc, real_t up)
 {
     const real_t den = (real_t)(1.0 + (double)e);
     const real_t num = (real_t)(1.0 - (double)e);
-    value = value * (q / den);
+    value = value * C;
     const double a = 2.0 / (double)den;
     const real_t b = num / den;
     double t = a * (double)uc - (double)(b * up);
@@ -153,11 +154,12 @@ __device__ __forceinline__ real_t leapfrog_damped(real_t value, real_t q, real_t
     return (real_t)t;
 }
 
-// Leap-frog update where den == num == 1 (no damping at the point).
-__device__ __forceinline__ real_t leapfrog_plain(real_t value, real_t q, real_t uc,
+// Leap-frog update where den == num == 1 (no damping at the point; there
+// C == q).
+__device__ __forceinline__ real_t leapfrog_plain(real_t value, real_t C, real_t uc,
                                                  real_t up)
 {
-    value = value * q;
+    value = value * C;
     double t = 2.0 * (double)uc - (double)up;
     t = t + (double)value;
     return (real_t)t;
@@ -165,7 +167,9 @@ __device__ __forceinline__ real_t leapfrog_plain(real_t value, real_t q, real_t 
 
 // The kernel's run constants, kept small so they stay in SGPRs.
 struct StepArgs {
-    const real_t *uc, *up, *q, *e;
+    const real_t *uc, *up;
+    const real_t *qc;   // (dt2/s)/den per point
+    const real_t *e;
     real_t *un;
     int nz, nx, ny;
     unsigned pbytes;    // bytes per z plane (working type)
@@ -238,7 +242,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     const unsigned long long rem = (unsigned long long)(nz - zb) * PB;
     const rsrc_t r_uc = make_rsrc(A.uc + off0 / esz, rem);
     const rsrc_t r_up = make_rsrc(A.up + off0 / esz, rem);
-    const rsrc_t r_q = make_rsrc(A.q + off0 / esz, rem);
+    const rsrc_t r_q = make_rsrc(A.qc + off0 / esz, rem);
     const rsrc_t r_e = make_rsrc(A.e + off0 / esz, rem);
     const rsrc_t r_un = make_rsrc(A.un + off0 / esz, rem);
 
@@ -476,8 +480,8 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                     const long long p = (long long)i * nx * ny + (long long)j * ny + kk;
                     real_t v = out[q][e];
                     if (smask)
-                        v = add_sources_masked(*Pd, n, i, j, kk, nq[q][e],
-                                               damped ? ne[q][e] : (real_t)0, v, smask);
+                        v = add_sources_masked(*Pd, n, i, j, kk, Pd->q[p], Pd->e[p], v,
+                                               smask);
                     if (bc_here)
                         store_with_bc(*Pd, A.un, p, i, j, kk, v);
                     else

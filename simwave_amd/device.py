@@ -3,9 +3,11 @@
 ``DeviceForward`` keeps one experiment's wavefield slots, models and shot
 record in HBM and calls the library's device entry
 (``simwave_hip_forward_device_<variant>``, declared in include/simwave_hip.h)
-on a HIP stream, so repeated shots pay no host<->device traffic.  Device
-memory comes from PyTorch's caching allocator (plumbing only; every
-computation is in the HIP library).
+on a HIP stream, so repeated shots pay no host<->device traffic.  Its plan
+methods use the persistent-plan entries (``simwave_hip_plan_*``) to run a
+shot in chunks of time steps on any stream.  Device memory comes from
+PyTorch's caching allocator (plumbing only; every computation is in the HIP
+library).
 """
 import ctypes
 
@@ -73,6 +75,73 @@ class DeviceForward:
         self._fn.restype = ctypes.c_double
         self._fn.argtypes = argtypes + [ctypes.c_void_p, ctypes.c_uint]
         self._args = args
+        self._plan_create = getattr(lib, "simwave_hip_plan_create_" + self.variant)
+        self._plan_create.restype = ctypes.c_void_p
+        self._plan_create.argtypes = argtypes + [ctypes.c_void_p, ctypes.c_uint]
+        self._plan_run = getattr(lib, "simwave_hip_plan_run_" + self.variant)
+        self._plan_run.restype = ctypes.c_longlong
+        self._plan_run.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        self._plan_destroy = getattr(lib, "simwave_hip_plan_destroy_" + self.variant)
+        self._plan_destroy.restype = ctypes.c_int
+        self._plan_destroy.argtypes = [ctypes.c_void_p]
+        self._plan = None
+
+    @property
+    def device_bytes(self):
+        """HBM this shot holds while planned: slots, models, shot record and
+        the library's per-point arrays (q, damping term, (dt^2/s)/den and
+        the density copy)."""
+        total = sum(t.numel() * t.element_size() for t in self.tensors.values())
+        vel = self.tensors["velocity_model"]
+        return total + 4 * vel.numel() * vel.element_size()
+
+    @property
+    def steps(self):
+        return self.kwargs["end_timestep"] - self.kwargs["begin_timestep"] + 1
+
+    def plan(self, stream=None, flags=0):
+        """Prepares the shot for chunked runs (``run_steps``) on the device."""
+        torch = self._torch
+        if self._plan is not None:
+            raise RuntimeError("plan already created")
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        h = self._plan_create(*self._args, ctypes.c_void_p(stream.cuda_stream),
+                              ctypes.c_uint(flags))
+        if not h:
+            from .middleware import _last_error
+            raise RuntimeError("plan creation failed: {}".format(_last_error(self._lib)))
+        self._plan = h
+        return self
+
+    def run_steps(self, steps, stream=None):
+        """Enqueues the next `steps` time steps on `stream` (no wait);
+        returns the steps left."""
+        torch = self._torch
+        if self._plan is None:
+            raise RuntimeError("no plan (call plan() first)")
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        left = self._plan_run(ctypes.c_void_p(self._plan), ctypes.c_size_t(int(steps)),
+                              ctypes.c_void_p(stream.cuda_stream))
+        if left < 0:
+            from .middleware import _last_error
+            raise RuntimeError("plan run failed: {}".format(_last_error(self._lib)))
+        return left
+
+    def release_plan(self):
+        """Frees the plan's device memory once its enqueued work is done."""
+        if self._plan is not None:
+            h, self._plan = self._plan, None
+            if self._plan_destroy(ctypes.c_void_p(h)) != 0:
+                from .middleware import _last_error
+                raise RuntimeError("plan destroy failed: {}".format(_last_error(self._lib)))
+
+    def __del__(self):
+        try:
+            self.release_plan()
+        except Exception:
+            pass
 
     @property
     def updates_per_shot(self):

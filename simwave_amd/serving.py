@@ -1,0 +1,252 @@
+"""Continuous-batching shot server on one GPU.
+
+Shot requests (``Solver`` objects, or callables that build one) arrive
+while others run.  The server
+
+* admits requests in arrival order against an HBM budget (every admitted
+  shot keeps its wavefield slots, models, shot record and the library's
+  per-point arrays resident) and a cap on concurrent shots;
+* runs every admitted shot on its own HIP stream, driven by its own worker
+  thread through the library's persistent plan (``simwave_hip_plan_*``),
+  in chunks of time steps, so the kernels of different shots overlap on the
+  device and their host-side launch work runs in parallel (the ctypes calls
+  release the GIL);
+* frees a finished shot's memory at once, which admits the next waiting
+  request, and resolves the request's future with ``(u_full, shot_record)``
+  exactly as ``Solver.forward()`` returns them;
+* records per-request latency (arrival to result on the host).
+
+Every shot runs the same kernels as when it runs alone, with the same
+inputs, so its result is bit-identical to ``Solver.forward()``.
+"""
+import collections
+import threading
+import time
+from concurrent.futures import Future
+
+import numpy as np
+
+__all__ = ["ShotServer", "ServeStats", "percentile", "shot_bytes"]
+
+
+def percentile(values, q):
+    """q-th percentile (linear interpolation, numpy's default)."""
+    if not len(values):
+        return float("nan")
+    return float(np.percentile(np.asarray(values, dtype=np.float64), q))
+
+
+def shot_bytes(solver):
+    """HBM an admitted shot holds: wavefield slots, models, shot record and
+    the library's per-point arrays (q, damping term, (dt^2/s)/den, density
+    copy)."""
+    if hasattr(solver, "device_bytes"):           # test doubles
+        return float(solver.device_bytes)
+    sm = solver.space_model
+    npts = int(np.prod(sm.extended_shape))
+    esz = np.dtype(sm.dtype).itemsize
+    slots = solver.num_snapshots + 2
+    models = 3 if sm.density_model is not None else 2
+    rec = solver.time_model.timesteps * solver.receivers.count
+    return float((slots + models + 4) * npts * esz + rec * esz)
+
+
+class ServeStats:
+    """Per-request timings of a server: arrival, admission, completion
+    (time.perf_counter seconds) and interior grid-point updates."""
+
+    def __init__(self):
+        self.records = []
+
+    def add(self, **rec):
+        self.records.append(rec)
+
+    @property
+    def latencies(self):
+        return [r["done"] - r["arrival"] for r in self.records]
+
+    def summary(self):
+        if not self.records:
+            return {}
+        lat = self.latencies
+        first = min(r["arrival"] for r in self.records)
+        last = max(r["done"] for r in self.records)
+        updates = sum(r["updates"] for r in self.records)
+        return {
+            "requests": len(self.records),
+            "updates": int(updates),
+            "makespan_s": last - first,
+            "gpts": updates / (last - first) / 1e9 if last > first else float("nan"),
+            "latency_p50_s": percentile(lat, 50),
+            "latency_p99_s": percentile(lat, 99),
+            "queue_p50_s": percentile([r["admitted"] - r["arrival"] for r in self.records], 50),
+            "max_active": max(r["active_at_admission"] for r in self.records),
+        }
+
+
+class _Request:
+    __slots__ = ("solver", "arrival", "future", "admitted")
+
+    def __init__(self, solver, arrival):
+        self.solver = solver
+        self.arrival = arrival
+        self.future = Future()
+        self.admitted = None
+
+
+class ShotServer:
+    """Admits and runs forward shots concurrently on one device.
+
+    Parameters
+    ----------
+    hbm_budget : float
+        Bytes of device memory the admitted shots may hold together.
+    max_active : int
+        Shots running at once (one worker thread and stream each).
+    chunk_steps : int
+        Time steps a worker enqueues per plan call.
+    device : str
+    shot_factory : callable, optional
+        ``solver -> shot`` with the ``DeviceForward`` interface (``steps``,
+        ``updates_per_shot``, ``plan``, ``run_steps``, ``results``,
+        ``release_plan``); replaceable for tests, as is ``stream_factory``.
+    """
+
+    def __init__(self, hbm_budget=200e9, max_active=16, chunk_steps=64, device="cuda",
+                 shot_factory=None, stream_factory=None):
+        self.hbm_budget = float(hbm_budget)
+        self.max_active = int(max_active)
+        self.chunk_steps = int(chunk_steps)
+        self.device = device
+        self._shot_factory = shot_factory or self._device_shot
+        self._stream_factory = stream_factory or self._new_stream
+        self._cv = threading.Condition()
+        self._waiting = collections.deque()
+        self._active = 0
+        self._used = 0.0
+        self._closed = False
+        self._workers = []
+        self.stats = ServeStats()
+        self._sched = threading.Thread(target=self._schedule, name="shot-scheduler",
+                                       daemon=True)
+        self._sched.start()
+
+    # ------------------------------------------------------------ public
+    def submit(self, solver, arrival=None):
+        """Queues a shot; returns a Future of (u_full, shot_record)."""
+        req = _Request(solver, time.perf_counter() if arrival is None else arrival)
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("server is closed")
+            self._waiting.append(req)
+            self._cv.notify_all()
+        return req.future
+
+    def close(self, wait=True):
+        """Stops admitting new requests; with wait, drains the queue."""
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        if wait:
+            self._sched.join()
+            for t in list(self._workers):
+                t.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close(wait=True)
+
+    # ------------------------------------------------------------ internals
+    def _device_shot(self, solver):
+        from .device import DeviceForward
+        return DeviceForward(solver, device=self.device)
+
+    def _new_stream(self):
+        import torch
+        return torch.cuda.Stream(device=self.device)
+
+    def _schedule(self):
+        while True:
+            with self._cv:
+                while True:
+                    if self._waiting and self._active < self.max_active:
+                        break
+                    if self._closed and not self._waiting:
+                        return
+                    self._cv.wait()
+                req = self._waiting[0]
+            try:
+                if callable(req.solver) and not hasattr(req.solver, "space_model") \
+                        and not hasattr(req.solver, "device_bytes"):
+                    req.solver = req.solver()
+                need = shot_bytes(req.solver)
+            except Exception as exc:          # a bad request fails alone
+                with self._cv:
+                    self._waiting.popleft()
+                req.future.set_exception(exc)
+                continue
+            with self._cv:
+                if need > self.hbm_budget:
+                    self._waiting.popleft()
+                    req.future.set_exception(MemoryError(
+                        "shot needs %.3g GB, more than the %.3g GB budget"
+                        % (need / 1e9, self.hbm_budget / 1e9)))
+                    continue
+                while (self._used + need > self.hbm_budget and self._active > 0) or \
+                        self._active >= self.max_active:
+                    self._cv.wait()
+                self._waiting.popleft()
+                self._used += need
+                self._active += 1
+                active_now = self._active
+                req.admitted = time.perf_counter()
+            t = threading.Thread(target=self._work, name="shot-worker", daemon=True,
+                                 args=(req, need, active_now))
+            self._workers.append(t)
+            t.start()
+
+    def _work(self, req, need, active_now):
+        shot = None
+        try:
+            stream = self._stream_factory()
+            try:
+                import torch
+                ctx = torch.cuda.stream(stream)
+            except Exception:                  # test doubles: no torch stream
+                ctx = _NullContext()
+            with ctx:
+                shot = self._shot_factory(req.solver)
+            shot.plan(stream)
+            left = shot.steps
+            while left > 0:
+                left = shot.run_steps(self.chunk_steps, stream)
+            stream.synchronize()
+            result = shot.results()
+            shot.release_plan()
+            done = time.perf_counter()
+            self.stats.add(arrival=req.arrival, admitted=req.admitted, done=done,
+                           updates=shot.updates_per_shot, active_at_admission=active_now)
+            req.future.set_result(result)
+        except Exception as exc:
+            if shot is not None:
+                try:
+                    shot.release_plan()
+                except Exception:
+                    pass
+            req.future.set_exception(exc)
+        finally:
+            shot = None
+            with self._cv:
+                self._used -= need
+                self._active -= 1
+                self._cv.notify_all()
+
+
+class _NullContext:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
