@@ -179,7 +179,7 @@ def run_serving(args):
                 time.sleep(delay)
             futs.append(srv.submit(s, arrival=due))
         for i, f in enumerate(futs):
-            f.result(timeout=300)
+            f.result(timeout=150)
             log("serving: request %d/%d done" % (i + 1, len(futs)))
     summ = srv.stats.summary()
     log("serving: %s" % summ)
@@ -340,7 +340,7 @@ def main():
 
     import torch
     import simwave_amd as sw
-    from simwave_amd.device import DeviceForward, FLAG_TIME_STEPS
+    from simwave_amd.device import DeviceForward, FLAG_TIME_STEPS, FLAG_OVERLAP_RECEIVERS
 
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
@@ -382,7 +382,7 @@ def main():
     roof = None
     if not args.no_roofline:
         dev.reset()
-        dev.run(flags=FLAG_TIME_STEPS)
+        dev.run(flags=FLAG_TIME_STEPS | FLAG_OVERLAP_RECEIVERS)
         tm = dev.timings()
         avg = tm[4] / max(tm[5], 1)
         r = kw["space_order"] // 2

@@ -62,7 +62,9 @@ double simwave_hip_forward_2d_cd_f32(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference constant_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -136,7 +138,9 @@ double simwave_hip_forward_2d_cd_f64(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference constant_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -210,7 +214,9 @@ double simwave_hip_forward_2d_vd_f32(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference variable_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -284,7 +290,9 @@ double simwave_hip_forward_2d_vd_f64(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference variable_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_vd_f64(
     double *u, double *velocity, double *density, double *damp,
@@ -358,7 +366,9 @@ double simwave_hip_forward_3d_cd_f32(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference constant_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -432,7 +442,9 @@ double simwave_hip_forward_3d_cd_f64(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference constant_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -506,7 +518,9 @@ double simwave_hip_forward_3d_vd_f32(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference variable_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -580,7 +594,9 @@ double simwave_hip_forward_3d_vd_f64(
  * rest stay on the host); runs on the HIP stream `stream` (NULL: the null
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
- * generic kernels.  Same arguments as
+ * generic kernels, 4 run the receiver gather of large grids on a second
+ * stream of the library's (event-ordered; leave it off when many threads
+ * drive many streams).  Same arguments as
  * reference variable_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_vd_f64(
     double *u, double *velocity, double *density, double *damp,

@@ -85,11 +85,18 @@ double now_s()
 
 constexpr unsigned kFlagTimeSteps = 1u;    // accumulate per-launch step time
 constexpr unsigned kFlagGeneric = 2u;      // force the generic kernels
+// receiver gather on a second stream of the library's own, joined by event
+// waits (large grids only).  Opt-in: a caller that drives many streams from
+// many threads keeps each shot on its one stream instead, since event waits
+// between streams that share a hardware queue can then order into a cycle.
+constexpr unsigned kFlagOverlapReceivers = 4u;
 constexpr int kMaxFusedSources = 64;
 // cooperative receiver kernel: receivers per block and points per round
 // (LDS = kRecChunk * kRecPerBlock values)
 constexpr int kRecPerBlock = sizeof(real_t) == 4 ? 16 : 8;
 constexpr int kRecChunk = 768;
+// grids from this many points on run the receiver gather on a second stream
+constexpr long long kOverlapMinPoints = 1ll << 22;
 
 // All arguments of one forward call, normalised across variants.
 struct Args {
@@ -822,7 +829,11 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
         for (hipEvent_t &e : tev.t)
             hip_check(hipEventCreate(&e), "hipEventCreate");
     }
-    const bool overlap_rec = a.nrec && !env_flag("SIMWAVE_HIP_REC_SERIAL");
+    // the overlap costs three extra stream operations per step: worth it
+    // where a step kernel is long enough to hide the gather
+    const bool overlap_rec = a.nrec && (flags & kFlagOverlapReceivers) &&
+                             pr.npts >= kOverlapMinPoints &&
+                             !env_flag("SIMWAVE_HIP_REC_SERIAL");
     if (overlap_rec)
         ev.create();
     // waited[j]: st already waits for the receivers of entry i0 + j
@@ -1015,8 +1026,8 @@ double run_host(const Args &a)
         auto swap = [&](size_t x, size_t y) { std::swap(phys[x], phys[y]); };
 
         auto uploads = [&](size_t idx) { return !up_at[idx].empty(); };
-        time_loop(a, pr, plan, 0, plan.size(), rec_dev, a.n0 - 1, 0u, st, slot_ptr, before,
-                  after, swap, uploads);
+        time_loop(a, pr, plan, 0, plan.size(), rec_dev, a.n0 - 1, kFlagOverlapReceivers, st,
+                  slot_ptr, before, after, swap, uploads);
         hip_check(hipStreamSynchronize(st), "time loop");
         const double tl1 = now_s();
         t_loop = tl1 - tl0;

@@ -15,10 +15,11 @@ import numpy as np
 
 from .middleware import Middleware
 
-__all__ = ["DeviceForward", "FLAG_TIME_STEPS", "FLAG_GENERIC"]
+__all__ = ["DeviceForward", "FLAG_TIME_STEPS", "FLAG_GENERIC", "FLAG_OVERLAP_RECEIVERS"]
 
 FLAG_TIME_STEPS = 1   # accumulate the step kernel's time with HIP events
 FLAG_GENERIC = 2      # use the generic kernels instead of the fused one
+FLAG_OVERLAP_RECEIVERS = 4   # receiver gather of large grids on a second stream
 
 # arguments that live in device memory for the device entry
 _DEVICE_KEYS = ("u_full", "velocity_model", "density_model", "damping_mask",
@@ -157,7 +158,7 @@ class DeviceForward:
         """Zeroes the wavefield slots (a fresh shot)."""
         self.tensors["u_full"].zero_()
 
-    def run(self, stream=None, flags=0):
+    def run(self, stream=None, flags=FLAG_OVERLAP_RECEIVERS):
         """Runs one shot on `stream` (default: torch's current stream)."""
         torch = self._torch
         if stream is None:
