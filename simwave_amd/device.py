@@ -21,6 +21,8 @@ FLAG_TIME_STEPS = 1   # accumulate the step kernel's time with HIP events
 FLAG_GENERIC = 2      # use the generic kernels instead of the fused one
 FLAG_OVERLAP_RECEIVERS = 4   # receiver gather of large grids on a second stream
 
+_TORCH_DTYPE = {"float32": None, "float64": None}
+
 # arguments that live in device memory for the device entry
 _DEVICE_KEYS = ("u_full", "velocity_model", "density_model", "damping_mask",
                 "shot_record")
@@ -45,6 +47,7 @@ class DeviceForward:
 
     def __init__(self, solver, device="cuda", end_timestep=None):
         import torch
+        _TORCH_DTYPE.update(float32=torch.float32, float64=torch.float64)
 
         self._torch = torch
         self.solver = solver
@@ -63,6 +66,11 @@ class DeviceForward:
         for key in _DEVICE_KEYS:
             host = kwargs.get(key)
             if host is None:
+                continue
+            if key in ("u_full", "shot_record") and not host.any():
+                # the Solver's fresh slots and record are zeros: no upload
+                self.tensors[key] = torch.zeros(host.shape, device=self.device,
+                                                dtype=_TORCH_DTYPE[str(host.dtype)])
                 continue
             self.tensors[key] = torch.from_numpy(np.ascontiguousarray(host)).to(
                 self.device)

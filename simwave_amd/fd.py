@@ -9,14 +9,17 @@ offsets -p..p with p = (deriv + acc - 1) // 2, then rounds them to float64.
 For deriv in {1, 2} and even acc this is the (acc+1)-point centred stencil.
 """
 from fractions import Fraction
+from functools import lru_cache
 
 import numpy as np
 
 __all__ = ["coefficients", "half_coefficients", "calculate_dt"]
 
 
+@lru_cache(maxsize=None)
 def _centred_weights(derivative_order, space_order):
-    """Exact centred weights as a list of Fractions for offsets -p..p."""
+    """Exact centred weights as a tuple of Fractions for offsets -p..p
+    (cached: the exact solve is the frontend's costliest step)."""
     if derivative_order < 1:
         raise ValueError("derivative order must be >= 1")
     if space_order < 1:
@@ -42,7 +45,7 @@ def _centred_weights(derivative_order, space_order):
             if r != col and a[r][col] != 0:
                 f = a[r][col]
                 a[r] = [vr - f * vc for vr, vc in zip(a[r], a[col])]
-    return [a[i][npts] for i in range(npts)]
+    return tuple(a[i][npts] for i in range(npts))
 
 
 def coefficients(derivative_order, space_order):

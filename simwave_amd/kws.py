@@ -124,11 +124,11 @@ def points_and_values(grid_shape, positions, half_width):
     points = np.empty((count, 2 * dim), dtype=np.uint)
     points[:, 0::2] = begins
     points[:, 1::2] = ends
-    chunks = []
-    for c in range(count):
-        for axis in range(dim):
-            w, first, last = per_axis[axis]
-            chunks.append(np.asarray(w[c, first[c]:last[c] + 1],
-                                     dtype=np.float32))
-    values = np.concatenate(chunks) if chunks else np.zeros(0, np.float32)
+    # per position, per axis, the weights first..last: a (count, dim, span)
+    # gather in row-major order
+    k = np.arange(span)[None, :]
+    w_all = np.stack([w for w, _, _ in per_axis], axis=1).astype(np.float32)
+    keep = np.stack([(k >= first[:, None]) & (k <= last[:, None])
+                     for _, first, last in per_axis], axis=1)
+    values = w_all[keep]
     return points.reshape(-1), values

@@ -15,6 +15,38 @@ __all__ = ["SpaceModel", "TimeModel"]
 _BOUNDARY_NAMES = ("none", "null_dirichlet", "null_neumann")
 
 
+
+def _pad_edge(a, widths):
+    """np.pad(a, widths, mode='edge'): a copy with every face replicated
+    outwards, axis by axis (the corners follow, as in numpy)."""
+    a = np.asarray(a)
+    out = np.empty(tuple(n + w0 + w1 for n, (w0, w1) in zip(a.shape, widths)),
+                   dtype=a.dtype)
+    core = tuple(slice(w0, w0 + n) for n, (w0, _) in zip(a.shape, widths))
+    out[core] = a
+    for ax, (w0, w1) in enumerate(widths):
+        lo, hi = w0, w0 + a.shape[ax]
+        idx = [slice(None)] * a.ndim
+        if w0:
+            idx[ax] = slice(0, lo)
+            src = [slice(None)] * a.ndim
+            src[ax] = slice(lo, lo + 1)
+            out[tuple(idx)] = out[tuple(src)]
+        if w1:
+            idx[ax] = slice(hi, hi + w1)
+            src = [slice(None)] * a.ndim
+            src[ax] = slice(hi - 1, hi)
+            out[tuple(idx)] = out[tuple(src)]
+    return out
+
+
+def _pad_zero(a, widths):
+    """np.pad(a, widths) (constant zero)."""
+    out = np.zeros(tuple(n + w0 + w1 for n, (w0, w1) in zip(a.shape, widths)),
+                   dtype=a.dtype)
+    out[tuple(slice(w0, w0 + n) for n, (w0, _) in zip(a.shape, widths))] = a
+    return out
+
 class SpaceModel:
     """Grid, velocity/density models, damping layer and halo of a run.
 
@@ -200,8 +232,15 @@ class SpaceModel:
         mask = np.pad(np.zeros(self.shape, dtype=self.dtype),
                       pad_width=self.nbl_pad_width, mode="linear_ramp",
                       end_values=self.nbl_pad_width)
-        mask = (mask ** self.damping_polynomial_degree) * self.damping_alpha
-        return np.pad(mask, pad_width=self.halo_pad_width)
+        deg = self.damping_polynomial_degree
+        if isinstance(deg, (int, np.integer)) and deg >= 3:
+            # the same power, evaluated only where the ramp is non-zero
+            # (0**deg == 0); the interior is most of the grid
+            mask = np.power(mask, deg, out=np.zeros_like(mask), where=mask != 0)
+        else:
+            mask = mask ** deg
+        mask = mask * self.damping_alpha
+        return _pad_zero(mask, self.halo_pad_width)
 
     @property
     def extended_grid(self):
@@ -209,8 +248,12 @@ class SpaceModel:
                       pad_width=self.halo_pad_width)
 
     def _edge_extend(self, model):
-        return np.pad(np.pad(model, pad_width=self.nbl_pad_width, mode="edge"),
-                      pad_width=self.halo_pad_width, mode="edge")
+        """np.pad(np.pad(model, nbl, 'edge'), halo, 'edge') as one copy:
+        both are edge replication, so the result replicates the model's
+        edges over nbl + halo."""
+        widths = tuple((a[0] + b[0], a[1] + b[1])
+                       for a, b in zip(self.nbl_pad_width, self.halo_pad_width))
+        return _pad_edge(model, widths)
 
     @property
     def extended_velocity_model(self):

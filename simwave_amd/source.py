@@ -55,30 +55,34 @@ class Source:
 
     @property
     def grid_positions(self):
-        """(coord - min) / h per axis, in the model dtype (source.py:62-103)."""
+        """(coord - min) / h per axis (source.py:62-103).  Coordinates, box
+        and spacing are all held in the model dtype, so the subtraction and
+        the division are operations of that dtype, elementwise here."""
         sm = self.space_model
         box, h = sm.bounding_box, sm.grid_spacing
-        out = []
-        for coord in self.coordinates:
-            dim = len(coord)
-            if dim not in (2, 3):
-                raise Exception("Dimension %d not supported." % dim)
-            for a in range(dim):
-                if not box[2 * a] <= coord[a] <= box[2 * a + 1]:
-                    raise Exception("Coordinates %s out of bounds." % coord)
-            out.append(tuple((coord[a] - box[2 * a]) / h[a] for a in range(dim)))
-        return np.asarray(out, dtype=sm.dtype)
+        coords = np.asarray(self.coordinates, dtype=sm.dtype)
+        if coords.size == 0:
+            return np.zeros((0, sm.dimension), dtype=sm.dtype)
+        dim = coords.shape[1]
+        if dim not in (2, 3):
+            raise Exception("Dimension %d not supported." % dim)
+        lo = np.array([box[2 * a] for a in range(dim)], dtype=sm.dtype)
+        hi = np.array([box[2 * a + 1] for a in range(dim)], dtype=sm.dtype)
+        bad = ((coords < lo) | (coords > hi)).any(axis=1)
+        if bad.any():
+            raise Exception("Coordinates %s out of bounds."
+                            % (tuple(self.coordinates[int(np.argmax(bad))]),))
+        step = np.array([h[a] for a in range(dim)], dtype=sm.dtype)
+        return ((coords - lo) / step).astype(sm.dtype)
 
     @property
     def adjusted_grid_positions(self):
         """Grid positions shifted by the leading damping layer and halo
-        (source.py:105-121)."""
+        (source.py:105-121), added in the model dtype."""
         sm = self.space_model
-        origin = [sm.nbl[2 * a] + sm.halo_size[2 * a]
-                  for a in range(sm.dimension)]
-        shifted = [tuple(pos[a] + origin[a] for a in range(len(pos)))
-                   for pos in self.grid_positions]
-        return np.asarray(shifted, dtype=sm.dtype)
+        origin = np.array([sm.nbl[2 * a] + sm.halo_size[2 * a]
+                           for a in range(sm.dimension)], dtype=sm.dtype)
+        return (self.grid_positions + origin).astype(sm.dtype)
 
     @property
     def interpolated_points_and_values(self):

@@ -116,8 +116,8 @@ def test_fd_closed_forms(space_order):
     c2 = [-2 * sum(c2)] + c2
     c1 = [Fraction(0)] + [Fraction((-1) ** (k + 1) * f(m) ** 2, k * f(m - k) * f(m + k))
                           for k in range(1, m + 1)]
-    assert fd._centred_weights(2, space_order)[m:] == c2
-    assert fd._centred_weights(1, space_order)[m:] == c1
+    assert list(fd._centred_weights(2, space_order)[m:]) == c2
+    assert list(fd._centred_weights(1, space_order)[m:]) == c1
     assert np.array_equal(fd.half_coefficients(2, space_order),
                           np.array([float(c) for c in c2]))
     # antisymmetry / symmetry of the full stencils
@@ -431,3 +431,50 @@ def test_compiler_resolves_hip_library():
     from simwave_amd import build
     path = sw.Compiler().compile(3, "constant_density", "-DFLOAT", "forward")
     assert path == build.library_path(3, "cd", "f32")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_vectorised_positions_match_scalar_expressions(dtype):
+    """grid_positions / adjusted_grid_positions, evaluated on whole arrays,
+    equal the reference's per-coordinate scalar expressions
+    (source.py:62-121 of the reference: dtype scalars throughout)."""
+    rng = np.random.default_rng(3)
+    sm = sw.SpaceModel(bounding_box=(0, 4787.5, -13.3, 4000.7, 0.1, 999.9),
+                       grid_spacing=(12.5, 7.3, 3.3),
+                       velocity_model=np.full((20, 20, 20), 1500.0, dtype=dtype),
+                       space_order=8, dtype=dtype)
+    sm.config_boundary(damping_length=(100.0, 50.0, 70.0, 10.0, 33.0, 0.0))
+    coords = np.stack([rng.uniform(0, 4787.5, 2000), rng.uniform(-13.3, 4000.7, 2000),
+                       rng.uniform(0.1, 999.9, 2000)], 1)
+    src = sw.Source(sm, coordinates=coords)
+    box, h = sm.bounding_box, sm.grid_spacing
+    origin = [sm.nbl[2 * a] + sm.halo_size[2 * a] for a in range(3)]
+    ref = np.asarray([tuple((c[a] - box[2 * a]) / h[a] for a in range(3))
+                      for c in src.coordinates], dtype=dtype)
+    ref_adj = np.asarray([tuple(p[a] + origin[a] for a in range(3)) for p in ref],
+                         dtype=dtype)
+    assert np.array_equal(src.grid_positions, ref)
+    assert np.array_equal(src.adjusted_grid_positions, ref_adj)
+
+
+def test_points_and_values_layout_matches_per_position_calls():
+    """The vectorised gather lays weights out per position, per axis, as
+    concatenating single-position results does (source.py:140-159)."""
+    rng = np.random.default_rng(5)
+    shape = (61, 47, 73)
+    pos = np.stack([rng.uniform(5, n - 6, 40) for n in shape], 1).astype(np.float32)
+    pts, vals = kws.points_and_values(shape, pos, 4)
+    one = [kws.points_and_values(shape, pos[i:i + 1], 4) for i in range(len(pos))]
+    assert np.array_equal(pts, np.concatenate([p for p, _ in one]))
+    assert np.array_equal(vals, np.concatenate([v for _, v in one]))
+
+
+def test_padding_helpers_equal_numpy_pad():
+    from simwave_amd.model import _pad_edge, _pad_zero
+    rng = np.random.default_rng(0)
+    for shape in [(5, 7), (3, 4, 6), (1, 1), (2, 3, 1)]:
+        a = rng.standard_normal(shape).astype(np.float32)
+        for w in [((0, 0),) * len(shape), ((1, 2),) * len(shape),
+                  tuple((i, len(shape) - i + 1) for i in range(len(shape)))]:
+            assert np.array_equal(_pad_edge(a, w), np.pad(a, w, mode="edge"))
+            assert np.array_equal(_pad_zero(a, w), np.pad(a, w))
