@@ -91,6 +91,86 @@ def readme_2d(dtype=np.float32):
                 wavelet=sw.RickerWavelet(10.0, tm))
 
 
+def faulted_2d_vd(dtype=np.float32, seed=0):
+    """2D variable density, 1000 x 3000 at 5 m, space order 16, 60-cell
+    damping; 20 seeded dipping layers (1500-5500 m/s) offset by 3 seeded
+    normal faults; density from Gardner's relation rho = 310 v^0.25;
+    tf 3.0 s; 25 Hz Ricker at the surface centre; 3000 surface receivers;
+    saving_stride 10 (every 10th wavefield returned)."""
+    import simwave_amd as sw
+    nz, nx, h = 1000, 3000, 5.0
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[:, None] * h
+    x = np.arange(nx)[None, :] * h
+    dip = np.tan(np.deg2rad(rng.uniform(-8.0, 8.0)))
+    depth = z - dip * (x - x.mean())
+    for _ in range(3):                      # normal faults: the hanging wall drops
+        x0 = rng.uniform(0.2, 0.8) * nx * h
+        slope = np.tan(np.deg2rad(rng.uniform(55.0, 75.0)))
+        throw = rng.uniform(40.0, 150.0)
+        hanging = x > x0 + z / slope
+        depth = np.where(hanging, depth - throw, depth)
+    edges = np.sort(rng.uniform(0.0, nz * h, 19))
+    speeds = np.linspace(1500.0, 5500.0, 20)
+    vel = speeds[np.searchsorted(edges, depth)].astype(dtype)
+    den = (310.0 * vel.astype(np.float64) ** 0.25).astype(dtype)
+    sm = sw.SpaceModel(bounding_box=(0, (nz - 1) * h, 0, (nx - 1) * h),
+                       grid_spacing=(h, h), velocity_model=vel, density_model=den,
+                       space_order=16, dtype=dtype)
+    sm.config_boundary(damping_length=60 * h,
+                       boundary_condition=("null_neumann",) + ("null_dirichlet",) * 3,
+                       damping_polynomial_degree=3, damping_alpha=0.001)
+    tm = sw.TimeModel(space_model=sm, tf=3.0, saving_stride=10)
+    src = sw.Source(sm, coordinates=[(2 * h, (nx - 1) * h / 2)])
+    rec = sw.Receiver(sm, coordinates=[(2 * h, x) for x in
+                                       np.linspace(0.0, (nx - 1) * h, 3000)])
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(25.0, tm))
+
+
+def run_host_workload(args, builder, desc):
+    """Times Solver.forward() (the reference-compatible host entry: models
+    and every returned snapshot cross PCIe inside the call)."""
+    import torch
+    import simwave_amd as sw
+    t0 = time.time()
+    parts = builder()
+    solver = sw.Solver(**parts)
+    kw = solver.forward_arguments()
+    log("workload %s: grid %s, %d steps, %d snapshots, host prep %.1fs"
+        % (args.workload, kw["velocity_model"].shape, kw["end_timestep"],
+           kw["num_snapshots"], time.time() - t0))
+    r = kw["space_order"] // 2
+    upd = int(np.prod([n - 2 * r for n in kw["velocity_model"].shape])) * kw["end_timestep"]
+    for _ in range(args.warmup):
+        solver.forward()
+    times = []
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        u, rec = solver.forward()
+        times.append(time.perf_counter() - ts)
+        log("shot %d: %.3fs" % (i, times[-1]))
+    el = float(np.median(times))
+    out = {
+        "metric": METRIC, "value": round(upd / el / 1e9, 3), "unit": "Gpts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc,
+                   "grid": list(kw["velocity_model"].shape),
+                   "time_steps": int(kw["end_timestep"]),
+                   "snapshots": int(kw["num_snapshots"]), "updates_per_shot": upd,
+                   "path": "host entry (Solver.forward), PCIe-inclusive"},
+        "shot_latency_ms": {"p50": round(float(np.percentile(times, 50)) * 1e3, 2),
+                            "p99": round(float(np.percentile(times, 99)) * 1e3, 2),
+                            "shots": len(times), "load": "one shot at a time"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def serving_requests(count, seed=0, n2=(256, 1024), n3=(64, 160), steps=(100, 300)):
     """Seeded stream of shot requests for the serving workload: 2D or 3D
     (50/50), 2D grids n2[0]^2..n2[1]^2 and 3D grids n3[0]^3..n3[1]^3 at
@@ -229,6 +309,10 @@ WORKLOADS = {
     "2d_cd_so4_readme": (readme_2d,
                          "2D constant density so4 512^2 (README example)"),
     "serving": (None, "mixed shot stream through the continuous-batching server"),
+    "2d_vd_so16_stride10": (faulted_2d_vd,
+                            "2D variable density so16 1000x3000 (1136x3136 extended), "
+                            "5 m, 60-pt damping, faulted layers, tf 3.0 s, 3000 "
+                            "receivers, saving_stride 10 (host entry)"),
 }
 
 
@@ -347,6 +431,11 @@ def main():
         sys.exit(1)
     if args.workload == "serving":
         run_serving(args)
+        return
+    if args.workload == "2d_vd_so16_stride10":
+        builder, desc = WORKLOADS[args.workload]
+        run_host_workload(args, lambda: builder(
+            dtype=np.float32 if args.dtype == "f32" else np.float64), desc)
         return
     dtype = np.float32 if args.dtype == "f32" else np.float64
     builder, desc = WORKLOADS[args.workload]

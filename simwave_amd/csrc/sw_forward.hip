@@ -21,7 +21,12 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <condition_variable>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -931,7 +936,247 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
 }
 
 // --------------------------------------------------------- host entry
+// Copies between pageable and pinned host memory with a few threads: a
+// single thread moves about 10 GB/s, short of what the DMA engines do.
+class HostCopier {
+public:
+    explicit HostCopier(int nthreads)
+    {
+        for (int t = 0; t < nthreads; t++)
+            threads_.emplace_back([this] { work(); });
+    }
+    ~HostCopier()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : threads_)
+            t.join();
+    }
+    // Blocking parallel memcpy.
+    void copy(void *dst, const void *src, size_t bytes)
+    {
+        const size_t parts = threads_.size() + 1;
+        std::unique_lock<std::mutex> g(m_);
+        dst_ = static_cast<char *>(dst);
+        src_ = static_cast<const char *>(src);
+        bytes_ = bytes;
+        chunk_ = (bytes + parts - 1) / parts;
+        next_ = 0;
+        done_ = 0;
+        parts_ = parts;
+        gen_++;
+        g.unlock();
+        cv_.notify_all();
+        run_chunks();
+        g.lock();
+        done_cv_.wait(g, [&] { return done_ == parts_; });
+    }
+
+private:
+    void run_chunks()
+    {
+        for (;;) {
+            size_t part;
+            {
+                std::lock_guard<std::mutex> g(m_);
+                if (next_ >= parts_)
+                    return;
+                part = next_++;
+            }
+            const size_t lo = part * chunk_;
+            const size_t hi = std::min(bytes_, lo + chunk_);
+            if (lo < hi)
+                std::memcpy(dst_ + lo, src_ + lo, hi - lo);
+            std::lock_guard<std::mutex> g(m_);
+            if (++done_ == parts_)
+                done_cv_.notify_all();
+        }
+    }
+    void work()
+    {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_)
+                    return;
+                seen = gen_;
+            }
+            run_chunks();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex m_;
+    std::condition_variable cv_, done_cv_;
+    bool stop_ = false;
+    unsigned long long gen_ = 0;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t bytes_ = 0, chunk_ = 0, next_ = 0, done_ = 0, parts_ = 0;
+};
+
+// Downloads finished slots without stalling the time loop: a device slot is
+// copied on `ds` into one of a ring of pinned staging buffers, and a thread
+// moves it from there into the caller's array once the copy has landed.
+class SlotDownloader {
+public:
+    SlotDownloader(size_t bytes, int depth, hipStream_t ds)
+        : bytes_(bytes), ds_(ds), copier_(4)
+    {
+        stage_.assign(depth, nullptr);
+        done_.assign(depth, nullptr);
+        busy_.assign(depth, false);
+        for (int k = 0; k < depth; k++) {
+            hip_check(hipHostMalloc((void **)&stage_[k], bytes, hipHostMallocDefault),
+                      "hipHostMalloc (staging)");
+            hip_check(hipEventCreateWithFlags(&done_[k], hipEventDisableTiming),
+                      "hipEventCreate");
+        }
+        thread_ = std::thread([this] { work(); });
+    }
+    ~SlotDownloader()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        thread_.join();
+        for (size_t k = 0; k < stage_.size(); k++) {
+            if (done_[k])
+                (void)hipEventDestroy(done_[k]);
+            if (stage_[k])
+                (void)hipHostFree(stage_[k]);
+        }
+    }
+    // Enqueues the copy of device slot `src` (complete once `after` fires)
+    // into `dst`; blocks while every staging buffer is in flight.
+    void push(const real_t *src, real_t *dst, hipEvent_t after)
+    {
+        int k;
+        {
+            std::unique_lock<std::mutex> g(m_);
+            cv_.wait(g, [&] { return !busy_[ring_] || !error_.empty(); });
+            if (!error_.empty())
+                fail("%s", error_.c_str());
+            k = ring_;
+            ring_ = (ring_ + 1) % (int)stage_.size();
+            busy_[k] = true;
+        }
+        hip_check(hipStreamWaitEvent(ds_, after, 0), "hipStreamWaitEvent");
+        hip_check(hipMemcpyAsync(stage_[k], src, bytes_, hipMemcpyDeviceToHost, ds_),
+                  "download slot");
+        hip_check(hipEventRecord(done_[k], ds_), "hipEventRecord");
+        {
+            std::lock_guard<std::mutex> g(m_);
+            jobs_.push_back({k, dst});
+        }
+        cv_.notify_all();
+    }
+    // Waits until every pushed slot is in the caller's array.
+    void drain()
+    {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] {
+            if (!error_.empty())
+                return true;
+            if (!jobs_.empty())
+                return false;
+            for (bool b : busy_)
+                if (b)
+                    return false;
+            return true;
+        });
+        if (!error_.empty())
+            fail("%s", error_.c_str());
+    }
+
+private:
+    struct Job {
+        int k;
+        real_t *dst;
+    };
+    void work()
+    {
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || !jobs_.empty(); });
+                if (jobs_.empty())
+                    return;
+                j = jobs_.front();
+                jobs_.pop_front();
+            }
+            const hipError_t e = hipEventSynchronize(done_[j.k]);
+            if (e == hipSuccess)
+                copier_.copy(j.dst, stage_[j.k], bytes_);
+            {
+                std::lock_guard<std::mutex> g(m_);
+                if (e != hipSuccess && error_.empty())
+                    error_ = std::string("slot download: ") + hipGetErrorString(e);
+                busy_[j.k] = false;
+            }
+            cv_.notify_all();
+        }
+    }
+    size_t bytes_;
+    hipStream_t ds_;
+    HostCopier copier_;
+    std::vector<real_t *> stage_;
+    std::vector<hipEvent_t> done_;
+    std::vector<bool> busy_;
+    int ring_ = 0;
+    std::deque<Job> jobs_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    bool stop_ = false;
+    std::string error_;
+    std::thread thread_;
+};
+
+// Element indices of the halo frame (points outside [r, n-r) on some axis)
+// of one slot, in ascending order.
+std::vector<unsigned long long> frame_indices(const Params &P)
+{
+    std::vector<unsigned long long> out;
+    const long long r = P.r, nz = P.nz, nx = P.nx, ny = P.ny;
+    const bool d3 = SW_DIM == 3;
+    for (long long i = 0; i < nz; i++)
+        for (long long j = 0; j < nx; j++) {
+            const unsigned long long row = (unsigned long long)(i * nx + j) * (d3 ? ny : 1);
+            const bool whole = i < r || i >= nz - r || (d3 && (j < r || j >= nx - r));
+            if (!d3) {
+                if (whole || j < r || j >= nx - r)
+                    out.push_back(row);
+                continue;
+            }
+            if (whole) {
+                for (long long k = 0; k < ny; k++)
+                    out.push_back(row + k);
+            } else {
+                for (long long k = 0; k < r; k++)
+                    out.push_back(row + k);
+                for (long long k = ny - r; k < ny; k++)
+                    out.push_back(row + k);
+            }
+        }
+    return out;
+}
+
 // Arrays are host pointers (the reference contract).  Returns wall seconds.
+//
+// Slot residency: a logical slot lives on the device from its first to its
+// last use.  A slot first used as a step's input is uploaded whole; one
+// first used as the step's output only needs its halo frame (the step
+// writes every interior point), which is uploaded packed and scattered.
+// A slot is downloaded after its last use through SlotDownloader, so the
+// transfers overlap the time loop; its device buffer is reused only after
+// its download has fired.
 double run_host(const Args &a)
 {
     const double t0 = now_s();
@@ -943,16 +1188,22 @@ double run_host(const Args &a)
     if (ndev < 1)
         fail("no HIP device visible");
     keep_pool_memory();
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr, ds = nullptr;
     hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
     struct StreamGuard {
         hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
+        ~StreamGuard()
+        {
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+        }
     } sg{st};
+    hip_check(hipStreamCreateWithFlags(&ds, hipStreamNonBlocking), "hipStreamCreate");
+    StreamGuard sgd{ds};
 
     std::vector<StepSlots> plan = plan_slots(a);
-    const size_t D = SW_DIM;
-    (void)D;
     // slot residency: first and last step index of every logical slot
     std::vector<long long> first(a.num_snapshots, -1), last(a.num_snapshots, -1);
     auto touch = [&](size_t slot, size_t idx) {
@@ -986,38 +1237,116 @@ double run_host(const Args &a)
 
         std::vector<real_t *> phys(a.num_snapshots, nullptr);
         std::vector<real_t *> pool;
+        std::map<real_t *, hipEvent_t> free_after;   // buffer -> download done
         auto acquire = [&]() -> real_t * {
             if (!pool.empty()) {
                 real_t *p = pool.back();
                 pool.pop_back();
+                auto it = free_after.find(p);
+                if (it != free_after.end()) {
+                    hip_check(hipStreamWaitEvent(st, it->second, 0), "hipStreamWaitEvent");
+                    free_after.erase(it);
+                }
                 return p;
             }
             return arena.alloc<real_t>(npts);
         };
         // bucket slots by the step that uploads / downloads them
         std::vector<std::vector<size_t>> up_at(plan.size()), down_at(plan.size());
+        bool any_frame = false;
         for (size_t sl = 0; sl < a.num_snapshots; sl++) {
             if (first[sl] < 0)
                 continue;
             up_at[first[sl]].push_back(sl);
             down_at[last[sl]].push_back(sl);
+            const StepSlots &s = plan[first[sl]];
+            any_frame |= s.next == sl && s.prev != sl && s.cur != sl;
         }
+        // halo frame: indices on the device, packed values staged pinned
+        std::vector<unsigned long long> fidx;
+        unsigned long long *d_fidx = nullptr;
+        real_t *d_fval = nullptr, *h_fval = nullptr;
+        hipEvent_t fval_free = nullptr;
+        struct PinnedGuard {
+            real_t *p = nullptr;
+            hipEvent_t e = nullptr;
+            ~PinnedGuard()
+            {
+                if (e)
+                    (void)hipEventDestroy(e);
+                if (p)
+                    (void)hipHostFree(p);
+            }
+        } pg;
+        if (any_frame) {
+            fidx = frame_indices(pr.P);
+            d_fidx = arena.alloc<unsigned long long>(fidx.size());
+            d_fval = arena.alloc<real_t>(fidx.size());
+            hip_check(hipMemcpyAsync(d_fidx, fidx.data(), fidx.size() * sizeof(fidx[0]),
+                                     hipMemcpyHostToDevice, st), "upload frame indices");
+            hip_check(hipHostMalloc((void **)&h_fval, fidx.size() * sizeof(real_t),
+                                    hipHostMallocDefault), "hipHostMalloc (frame)");
+            pg.p = h_fval;
+            hip_check(hipEventCreateWithFlags(&fval_free, hipEventDisableTiming),
+                      "hipEventCreate");
+            pg.e = fval_free;
+            hip_check(hipEventRecord(fval_free, st), "hipEventRecord");
+        }
+        hipEvent_t slot_done = nullptr;
+        hip_check(hipEventCreateWithFlags(&slot_done, hipEventDisableTiming), "hipEventCreate");
+        struct EventGuard {
+            hipEvent_t e;
+            ~EventGuard() { (void)hipEventDestroy(e); }
+        } eg{slot_done};
+        std::vector<hipEvent_t> buf_events;    // one per recycled-buffer download
+        struct EventsGuard {
+            std::vector<hipEvent_t> &v;
+            ~EventsGuard()
+            {
+                for (hipEvent_t e : v)
+                    (void)hipEventDestroy(e);
+            }
+        } evg{buf_events};
+        SlotDownloader down(slot_bytes, 3, ds);
         hip_check(hipStreamSynchronize(st), "prepare");
         t_prep = now_s() - t0;
         const double tl0 = now_s();
 
         auto before = [&](size_t idx) {
+            const StepSlots &s = plan[idx];
             for (size_t sl : up_at[idx]) {
                 real_t *d = acquire();
-                hip_check(hipMemcpyAsync(d, a.u + sl * npts, slot_bytes,
-                                         hipMemcpyHostToDevice, st), "upload slot");
+                const real_t *src = a.u + sl * npts;
+                if (s.next == sl && s.prev != sl && s.cur != sl) {
+                    hip_check(hipEventSynchronize(fval_free), "hipEventSynchronize");
+                    const size_t nf = fidx.size();
+                    for (size_t i = 0; i < nf; i++)
+                        h_fval[i] = src[fidx[i]];
+                    hip_check(hipMemcpyAsync(d_fval, h_fval, nf * sizeof(real_t),
+                                             hipMemcpyHostToDevice, st), "upload frame");
+                    hipLaunchKernelGGL(sw::k_scatter,
+                                       dim3((unsigned)std::min<size_t>((nf + 255) / 256, 4096)),
+                                       dim3(256), 0, st, d, d_fidx, d_fval, (long long)nf);
+                    hip_check(hipEventRecord(fval_free, st), "hipEventRecord");
+                } else {
+                    hip_check(hipMemcpyAsync(d, src, slot_bytes, hipMemcpyHostToDevice, st),
+                              "upload slot");
+                }
                 phys[sl] = d;
             }
         };
         auto after = [&](size_t idx) {
+            if (down_at[idx].empty())
+                return;
+            hip_check(hipEventRecord(slot_done, st), "hipEventRecord");
             for (size_t sl : down_at[idx]) {
-                hip_check(hipMemcpyAsync(a.u + sl * npts, phys[sl], slot_bytes,
-                                         hipMemcpyDeviceToHost, st), "download slot");
+                down.push(phys[sl], a.u + sl * npts, slot_done);
+                // the buffer may be rewritten once its download has fired
+                hipEvent_t e = nullptr;
+                hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+                buf_events.push_back(e);
+                hip_check(hipEventRecord(e, ds), "hipEventRecord");
+                free_after[phys[sl]] = e;
                 pool.push_back(phys[sl]);
                 phys[sl] = nullptr;
             }
@@ -1036,6 +1365,7 @@ double run_host(const Args &a)
                                      rows * a.nrec * sizeof(real_t),
                                      hipMemcpyDeviceToHost, st), "download receivers");
         hip_check(hipStreamSynchronize(st), "download");
+        down.drain();
         t_down = now_s() - tl1;
         g_timings[6] = pr.fused ? 1.0 : 0.0;
 #if SW_DIM == 3

@@ -871,6 +871,15 @@ __global__ void k_zero_u32(unsigned *p, long long n)
 }
 
 // Exchange two slots (the reference's even-stride swap, wave.c:578-611).
+// dst[idx[i]] = val[i]: places the halo frame of a slot uploaded packed.
+__global__ void k_scatter(real_t *__restrict__ dst, const unsigned long long *__restrict__ idx,
+                          const real_t *__restrict__ val, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[idx[i]] = val[i];
+}
+
 __global__ void k_swap(real_t *a, real_t *b, long long npts)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;

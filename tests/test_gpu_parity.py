@@ -149,3 +149,50 @@ def test_parity_tiled_configs_r4(gpu, name, tile, order):
 def test_parity_tiled_configs(gpu, name, tile, order):
     """The tile configurations the other radii carry."""
     _tiled(name, tile, order)
+
+
+HOST_ENTRY = {
+    "2d_stride0": dict(dim=2, n=40, stride=0, tf=0.06, damping=0.0,
+                       bc=("null_neumann", "null_dirichlet", "none", "null_neumann"),
+                       src=[(0.0, 200.0)]),
+    "2d_stride3": dict(dim=2, n=40, stride=3, tf=0.06),
+    "2d_stride4": dict(dim=2, n=36, stride=4, tf=0.06, density=True),
+    "3d_stride1": dict(dim=3, n=22, stride=1, tf=0.03, space_order=8),
+    "3d_stride2": dict(dim=3, n=24, stride=2, tf=0.04, damping=0.0,
+                       bc=("null_neumann",) * 6, src=[(0.0, 100.0, 120.0)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(HOST_ENTRY))
+def test_host_entry_keeps_initial_halo_values(gpu, name):
+    """The host entry uploads only the halo frame of slots first written by
+    a step; with random initial contents in every slot (the reference
+    contract allows any), the result must still equal the oracle's."""
+    import ctypes
+    import simwave_amd as sw
+    from simwave_amd.middleware import Middleware
+
+    print("case", name, "(host entry, random initial slots)", flush=True)
+    parts = cases.make_case(**HOST_ENTRY[name])()
+    solver = sw.Solver(**parts)
+    kw = Middleware.normalize(**solver.forward_arguments())
+    rng = np.random.default_rng(11)
+    kw["u_full"] = rng.standard_normal(kw["u_full"].shape).astype(kw["u_full"].dtype)
+    outs = []
+    for kind in ("hip", "oracle"):
+        k2 = dict(kw)
+        k2["u_full"] = kw["u_full"].copy()
+        k2["shot_record"] = np.zeros_like(kw["shot_record"])
+        path = harness.LibraryCompiler(kind).compile(
+            kw["velocity_model"].ndim,
+            "variable_density" if kw.get("density_model") is not None else "constant_density",
+            "-DFLOAT" if kw["velocity_model"].dtype == np.float32 else "-DDOUBLE", "forward")
+        lib = ctypes.CDLL(path)
+        argtypes, args = Middleware.ordered(k2)
+        lib.forward.argtypes = argtypes
+        lib.forward.restype = ctypes.c_double
+        assert lib.forward(*args) >= 0
+        outs.append((k2["u_full"], k2["shot_record"]))
+    (u_h, r_h), (u_o, r_o) = outs
+    assert np.array_equal(u_h, u_o)
+    assert np.array_equal(r_h, r_o)
