@@ -110,6 +110,34 @@ void *simwave_hip_plan_create_2d_cd_f32(
 long long simwave_hip_plan_run_2d_cd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density).
+ * Arguments of simwave_hip_forward_device_2d_cd_f32 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_2d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const float *rec_adjoint, double *grad_velocity, float *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -186,6 +214,34 @@ void *simwave_hip_plan_create_2d_cd_f64(
 long long simwave_hip_plan_run_2d_cd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density).
+ * Arguments of simwave_hip_forward_device_2d_cd_f64 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_2d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const double *rec_adjoint, double *grad_velocity, double *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -262,6 +318,34 @@ void *simwave_hip_plan_create_2d_vd_f32(
 long long simwave_hip_plan_run_2d_vd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
+ * Arguments of simwave_hip_forward_device_2d_vd_f32 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_2d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const float *rec_adjoint, double *grad_velocity, float *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -338,6 +422,34 @@ void *simwave_hip_plan_create_2d_vd_f64(
 long long simwave_hip_plan_run_2d_vd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
+ * Arguments of simwave_hip_forward_device_2d_vd_f64 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_2d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const double *rec_adjoint, double *grad_velocity, double *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -414,6 +526,34 @@ void *simwave_hip_plan_create_3d_cd_f32(
 long long simwave_hip_plan_run_3d_cd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density).
+ * Arguments of simwave_hip_forward_device_3d_cd_f32 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_3d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const float *rec_adjoint, double *grad_velocity, float *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -490,6 +630,34 @@ void *simwave_hip_plan_create_3d_cd_f64(
 long long simwave_hip_plan_run_3d_cd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density).
+ * Arguments of simwave_hip_forward_device_3d_cd_f64 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_3d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const double *rec_adjoint, double *grad_velocity, double *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -566,6 +734,34 @@ void *simwave_hip_plan_create_3d_vd_f32(
 long long simwave_hip_plan_run_3d_vd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
+ * Arguments of simwave_hip_forward_device_3d_vd_f32 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_3d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const float *rec_adjoint, double *grad_velocity, float *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
@@ -642,6 +838,34 @@ void *simwave_hip_plan_create_3d_vd_f64(
 long long simwave_hip_plan_run_3d_vd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
+/* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
+ * Arguments of simwave_hip_forward_device_3d_vd_f64 (device arrays; stride 0,
+ * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
+ * adjoint of the shot record [T][num_receivers] (device); grad_velocity,
+ * float64 [extended grid] (device), set to the gradient with respect to
+ * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
+ * the gradient with respect to the wavelet samples; checkpoint_every, the
+ * forward states saved every that many steps and recomputed per segment.
+ * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
+ * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * Also writes the shot record.  Returns wall seconds, or -1. */
+double simwave_hip_gradient_3d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    const double *rec_adjoint, double *grad_velocity, double *grad_wavelet,
+    size_t checkpoint_every, void *stream, unsigned flags);
 /* Host-only helper (no device work, no reference counterpart): the
  * fixed-divisor parameters the kernels use for the divisor b (a squared
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe

@@ -1,0 +1,294 @@
+// Adjoint (reverse-time) kernels of the constant-density forward operator.
+//
+// The forward step n maps (u^{n-1}, u^n) to u^{n+1} (cd-3d wave.c:142-480):
+//   interior p:  u^{n+1}_p = a_p u^n_p - b_p u^{n-1}_p + c_p (L u^n)_p
+//   sources:     u^{n+1}_p += c_p kws_{s,p} w_s^{n-1}
+//   boundaries:  per face line, Dirichlet zeroes and Neumann copies
+// with den = 1 + e, a = 2/den, b = (1 - e)/den, c = q/den, q = dt^2 v^2,
+// e = damp dt v^2 / 2, and receivers d^n = K u^n (wave.c:499-566).  These
+// kernels apply the transpose of that linear map, step by step backwards,
+// and accumulate the derivative of the data with respect to the velocity:
+//   da/dv = db/dv = -4 e / (v den^2),  dc/dv = 2 q / (v den^2).
+// L is the stencil sum over the axes of c2-weighted pairs divided by h^2;
+// it is symmetric, so its transpose is the same stencil evaluated on a
+// field that is zero outside the interior.
+#pragma once
+#include "sw_kernels.hpp"
+
+namespace sw {
+
+// Stencil of `f` at point (i, j, k), treating points outside the grid as
+// zero (the transpose of L applied to a field supported on the interior).
+__device__ __forceinline__ real_t stencil_clipped(const Params &P, const real_t *f, int i,
+                                                  int j, int k)
+{
+    const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
+    const int r = P.r;
+    const real_t c0 = f[p] * P.c2[0];
+    real_t sz = c0, sx = c0;
+#if SW_DIM == 3
+    real_t sy = c0;
+#endif
+    for (int m = 1; m <= r; m++) {
+        const real_t cm = P.c2[m];
+        real_t t = 0;
+        if (i + m < P.nz) t += f[p + m * P.sz];
+        if (i - m >= 0) t += f[p - m * P.sz];
+        sz += cm * t;
+        t = 0;
+        if (j + m < P.nx) t += f[p + m * P.sx];
+        if (j - m >= 0) t += f[p - m * P.sx];
+        sx += cm * t;
+#if SW_DIM == 3
+        t = 0;
+        if (k + m < P.ny) t += f[p + m];
+        if (k - m >= 0) t += f[p - m];
+        sy += cm * t;
+#endif
+    }
+#if SW_DIM == 3
+    return sy / P.dy2 + sx / P.dx2 + sz / P.dz2;
+#else
+    return sx / P.dx2 + sz / P.dz2;
+#endif
+}
+
+__device__ __forceinline__ void decode(const Params &P, long long p, int &i, int &j, int &k)
+{
+    i = (int)(p / P.sz);
+    const long long rem = p - (long long)i * P.sz;
+#if SW_DIM == 3
+    j = (int)(rem / P.sx);
+    k = (int)(rem - (long long)j * P.sx);
+#else
+    j = (int)rem;
+    k = 0;
+#endif
+}
+
+__device__ __forceinline__ bool interior(const Params &P, int i, int j, int k)
+{
+    const int r = P.r;
+    bool in = i >= r && i < P.nz - r && j >= r && j < P.nx - r;
+#if SW_DIM == 3
+    in = in && k >= r && k < P.ny - r;
+#else
+    (void)k;
+#endif
+    return in;
+}
+
+// ubar += K^T res: each receiver spreads its residual over its box.
+__global__ void __launch_bounds__(256)
+k_adj_receivers(Params P, const Box *__restrict__ rbox, const real_t *__restrict__ rw,
+                int nrec, const real_t *__restrict__ res, real_t *ubar)
+{
+    const int q = blockIdx.x;
+    if (q >= nrec)
+        return;
+    const Box b = rbox[q];
+    const real_t rv = res[q];
+    if (rv == (real_t)0)
+        return;
+    const int cz = b.hi[0] - b.lo[0] + 1, cx = b.hi[1] - b.lo[1] + 1;
+#if SW_DIM == 3
+    const int cy = b.hi[2] - b.lo[2] + 1;
+#else
+    const int cy = 1;
+#endif
+    for (int t = threadIdx.x; t < cz * cx * cy; t += blockDim.x) {
+        const int ii = t / (cx * cy), rem = t - ii * cx * cy;
+        const int jj = rem / cy, kk = rem - jj * cy;
+        const int i = b.lo[0] + ii, j = b.lo[1] + jj;
+#if SW_DIM == 3
+        const int k = b.lo[2] + kk;
+#else
+        const int k = 0;
+        (void)kk;
+#endif
+        const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
+        atomicAdd(&ubar[p], box_weight(b, rw, i, j, k) * rv);
+    }
+}
+
+// Transpose of one boundary phase (k_bc_phase): per face line, the after
+// face's Neumann copies, then its Dirichlet zero, then the same for the
+// before face.
+__global__ void k_adj_bc_phase(Params P, real_t *ub, int axis)
+{
+    const int r = P.r;
+    const long long li = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long base, st;
+    int n, before, after;
+#if SW_DIM == 3
+    const long long Iy = P.ny - 2 * r, Ix = P.nx - 2 * r, Iz = P.nz - 2 * r;
+    if (axis == 2) {
+        if (li >= Iz * Ix) return;
+        const int j = r + (int)(li % Ix), i = r + (int)(li / Ix);
+        base = (long long)i * P.sz + (long long)j * P.sx; st = 1; n = P.ny;
+        before = P.bc[4]; after = P.bc[5];
+    } else if (axis == 1) {
+        if (li >= Iz * Iy) return;
+        const int k = r + (int)(li % Iy), i = r + (int)(li / Iy);
+        base = (long long)i * P.sz + k; st = P.sx; n = P.nx;
+        before = P.bc[2]; after = P.bc[3];
+    } else {
+        if (li >= Ix * Iy) return;
+        const int k = r + (int)(li % Iy), j = r + (int)(li / Iy);
+        base = (long long)j * P.sx + k; st = P.sz; n = P.nz;
+        before = P.bc[0]; after = P.bc[1];
+    }
+#else
+    const long long Ix = P.nx - 2 * r, Iz = P.nz - 2 * r;
+    if (axis == 2) {
+        if (li >= Iz) return;
+        base = (long long)(r + li) * P.sz; st = 1; n = P.nx;
+        before = P.bc[2]; after = P.bc[3];
+    } else {
+        if (li >= Ix) return;
+        base = r + li; st = P.sz; n = P.nz;
+        before = P.bc[0]; after = P.bc[1];
+    }
+#endif
+    const long long lo = base + (long long)r * st;
+    const long long hi = base + (long long)(n - r - 1) * st;
+    if (after == 2)
+        for (int m = 1; m <= r; m++) {
+            ub[hi - m * st] += ub[hi + m * st];
+            ub[hi + m * st] = 0;
+        }
+    if (after == 1)
+        ub[hi] = 0;
+    if (before == 2)
+        for (int m = 1; m <= r; m++) {
+            ub[lo + m * st] += ub[lo - m * st];
+            ub[lo - m * st] = 0;
+        }
+    if (before == 1)
+        ub[lo] = 0;
+}
+
+// W = c * ubar on the interior, 0 elsewhere.
+__global__ void k_adj_cw(Params P, const real_t *__restrict__ ub, real_t *__restrict__ W,
+                         long long npts)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
+         p += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode(P, p, i, j, k);
+        real_t w = 0;
+        if (interior(P, i, j, k)) {
+            const double den = 1.0 + (double)P.e[p];
+            w = (real_t)((double)P.q[p] / den * (double)ub[p]);
+        }
+        W[p] = w;
+    }
+}
+
+// The transpose of the interior update of step n (k = n+1) and the
+// velocity derivative at every point:
+//   ub1 += a w + L^T W,  ub2 -= b w  (w = ub, the adjoint of u^{n+1}),
+//   grad += w (da (u1 - u2) + dc L u1)  with u1 = u^n, u2 = u^{n-1}.
+// With `u0` set (image mode) grad collects the zero-lag cross-correlation
+// w u^{n+1} (u0 = u^{n+1}) instead.
+__global__ void k_adj_update(Params P, const real_t *__restrict__ vel,
+                             const real_t *__restrict__ ub, const real_t *__restrict__ W,
+                             real_t *__restrict__ ub1, real_t *__restrict__ ub2,
+                             const real_t *__restrict__ u1, const real_t *__restrict__ u2,
+                             const real_t *__restrict__ u0, double *__restrict__ grad,
+                             long long npts)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
+         p += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode(P, p, i, j, k);
+        const real_t lt = stencil_clipped(P, W, i, j, k);
+        if (!interior(P, i, j, k)) {
+            ub1[p] += lt;
+            continue;
+        }
+        const real_t w = ub[p];
+        const double e = (double)P.e[p], q = (double)P.q[p], v = (double)vel[p];
+        const double den = 1.0 + e;
+        const double a = 2.0 / den, b = (1.0 - e) / den;
+        ub1[p] = (real_t)((double)ub1[p] + a * (double)w + (double)lt);
+        ub2[p] = (real_t)((double)ub2[p] - b * (double)w);
+        if (u0) {
+            grad[p] += (double)w * (double)u0[p];
+        } else if (w != (real_t)0) {
+            const double da = -4.0 * e / (v * den * den);
+            const double dc = 2.0 * q / (v * den * den);
+            const double lu = (double)stencil_clipped(P, u1, i, j, k);
+            grad[p] += (double)w * (da * ((double)u1[p] - (double)u2[p]) + dc * lu);
+        }
+    }
+}
+
+// Source terms of step n (k = n+1): the velocity derivative of
+// c kws w^{n-1} (skipped when `grad` is null) and the wavelet gradient
+// sum_p kws c_p ub_p.  One block per source; the wavelet gradient of source
+// s at step n goes to gw[s].
+__global__ void __launch_bounds__(256)
+k_adj_sources(Params P, const real_t *__restrict__ vel, const real_t *__restrict__ ub,
+              int n, double *__restrict__ grad, real_t *__restrict__ gw)
+{
+    __shared__ double part[256];
+    const int s = blockIdx.x;
+    if (s >= P.nsrc)
+        return;
+    const Box b = P.sbox[s];
+    const real_t amp = wavelet_amp(P, n, s);
+    const int cz = b.hi[0] - b.lo[0] + 1, cx = b.hi[1] - b.lo[1] + 1;
+#if SW_DIM == 3
+    const int cy = b.hi[2] - b.lo[2] + 1;
+#else
+    const int cy = 1;
+#endif
+    double acc = 0;
+    for (int t = threadIdx.x; t < cz * cx * cy; t += blockDim.x) {
+        const int ii = t / (cx * cy), rem = t - ii * cx * cy;
+        const int jj = rem / cy, kk = rem - jj * cy;
+        const int i = b.lo[0] + ii, j = b.lo[1] + jj;
+#if SW_DIM == 3
+        const int k = b.lo[2] + kk;
+#else
+        const int k = 0;
+        (void)kk;
+#endif
+        if (!interior(P, i, j, k))
+            continue;
+        const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
+        const double kws = (double)box_weight(b, P.sw, i, j, k);
+        const double e = (double)P.e[p], q = (double)P.q[p], v = (double)vel[p];
+        const double den = 1.0 + e;
+        const double w = (double)ub[p];
+        acc += kws * (q / den) * w;
+        if (grad && amp != (real_t)0 && w != 0.0)
+            atomicAdd(&grad[p], w * (2.0 * q / (v * den * den)) * kws * (double)amp);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && gw)
+        gw[s] = (real_t)part[0];
+}
+
+__global__ void k_fill(real_t *p, long long n, real_t v)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+}  // namespace sw

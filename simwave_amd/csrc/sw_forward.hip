@@ -16,6 +16,7 @@
 // device pointers.
 #include "sw_kernels.hpp"
 #include "sw_step3d.hpp"
+#include "sw_adjoint.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -95,6 +96,9 @@ constexpr unsigned kFlagGeneric = 2u;      // force the generic kernels
 // many threads keeps each shot on its one stream instead, since event waits
 // between streams that share a hardware queue can then order into a cycle.
 constexpr unsigned kFlagOverlapReceivers = 4u;
+// adjoint entry: accumulate the zero-lag cross-correlation image of the
+// forward and adjoint wavefields instead of the velocity gradient
+constexpr unsigned kAdjImage = 1u;
 constexpr int kMaxFusedSources = 64;
 // cooperative receiver kernel: receivers per block and points per round
 // (LDS = kRecChunk * kRecPerBlock values)
@@ -1472,6 +1476,163 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
     return total;
 }
 
+// ------------------------------------------------------------- adjoint
+// Shot record and its adjoint: runs the forward of a constant-density shot
+// (device arrays, as the device entry; stride 0, steps 1..T) saving the two
+// time levels (u^m, u^{m+1}) every `K` steps, then walks the checkpoint
+// segments backwards: recomputes each segment's states, then applies the
+// transpose of every step (sw_adjoint.hpp) to the adjoint of the shot
+// record `rec_adj` ([T][num_receivers], device).  Accumulates into
+// `grad` (float64, one value per extended grid point) the velocity
+// gradient and writes into `gw` ([T][num_sources], device, or null) the
+// wavelet gradient.
+double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *gw, size_t K,
+                   hipStream_t st, unsigned flags)
+{
+    const double t0 = now_s();
+    for (double &t : g_timings)
+        t = 0.0;
+    validate(a);
+#if SW_VD
+    (void)rec_adj; (void)grad; (void)gw; (void)K; (void)st; (void)flags;
+    fail("the adjoint covers constant density only");
+    return -1.0;
+#else
+    if (a.stride != 0)
+        fail("the adjoint needs saving_stride 0 (got %zu)", a.stride);
+    if (a.n0 != 1 || a.n1 < 1)
+        fail("the adjoint runs steps 1..T (got %zu..%zu)", a.n0, a.n1);
+    if (a.num_snapshots < 3)
+        fail("the adjoint needs the three wavefield slots");
+    if (K < 1)
+        fail("checkpoint interval must be >= 1");
+    if (!grad)
+        fail("NULL gradient array");
+    if (a.nrec && !rec_adj)
+        fail("NULL shot-record adjoint");
+    if (flags & ~kAdjImage)
+        fail("unknown adjoint flags 0x%x", flags & ~kAdjImage);
+    const bool image = flags & kAdjImage;
+    const int r = (int)(a.space_order / 2);
+    const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : (size_t)(3 * r + 1)};
+    for (int d = 0; d < SW_DIM; d++)
+        if (dims[d] < (size_t)(3 * r + 1))
+            fail("the adjoint needs at least 3r+1 points per axis");
+    keep_pool_memory();
+    DeviceArena arena(st);
+    Prepared pr = prepare(a, true, 0u, arena, st);
+    if (pr.halo_sources)
+        fail("the adjoint needs every source box inside the interior");
+    const Params &P = pr.P;
+    const size_t npts = (size_t)pr.npts;
+    const size_t T = a.n1;
+    const size_t ncp = (T + K - 1) / K;
+    const int nb = (int)std::min<long long>(((long long)npts + 255) / 256, 16384);
+    auto zero = [&](real_t *p, size_t count) {
+        hipLaunchKernelGGL(sw::k_fill, dim3(nb), dim3(256), 0, st, p, (long long)count,
+                           (real_t)0);
+    };
+    real_t *cp = arena.alloc<real_t>(2 * ncp * npts);
+    real_t *seg = arena.alloc<real_t>((K + 2) * npts);
+    real_t *ubar = arena.alloc<real_t>(3 * npts);
+    real_t *W = arena.alloc<real_t>(npts);
+    zero(seg, (K + 2) * npts);
+    zero(ubar, 3 * npts);
+    hipLaunchKernelGGL(sw::k_fill_f64, dim3(nb), dim3(256), 0, st, grad, (long long)npts);
+    if (gw && a.nsrc)
+        hipLaunchKernelGGL(sw::k_fill, dim3(1), dim3(256), 0, st, gw,
+                           (long long)(T * a.nsrc), (real_t)0);
+    hip_check(hipGetLastError(), "adjoint set-up");
+
+    // forward with checkpoints
+    std::vector<StepSlots> plan = plan_slots(a);
+    real_t *u = a.u;
+    auto slot_ptr = [&](size_t sl) { return u + sl * npts; };
+    auto save = [&](size_t idx) {
+        if (idx % K)
+            return;
+        const StepSlots &s = plan[idx];
+        const size_t j = idx / K;
+        hip_check(hipMemcpyAsync(cp + (2 * j) * npts, slot_ptr(s.prev), npts * sizeof(real_t),
+                                 hipMemcpyDeviceToDevice, st), "checkpoint");
+        hip_check(hipMemcpyAsync(cp + (2 * j + 1) * npts, slot_ptr(s.cur),
+                                 npts * sizeof(real_t), hipMemcpyDeviceToDevice, st),
+                  "checkpoint");
+    };
+    auto nop = [](size_t) {};
+    auto noswap = [](size_t, size_t) {};
+    auto no_uploads = [](size_t) { return false; };
+    time_loop(a, pr, plan, 0, plan.size(), a.receivers, 0, 0u, st, slot_ptr, save, nop, noswap,
+              no_uploads);
+    const double t_fwd = now_s();
+
+    // backward over the checkpoint segments
+    Args seg_args = a;
+    seg_args.nrec = 0;
+    const unsigned eb = (unsigned)std::min<size_t>((npts + 255) / 256, 16384);
+    for (size_t j = ncp; j-- > 0;) {
+        const size_t m = j * K;                   // levels m, m+1 saved
+        const size_t n_lo = m + 1, n_hi = std::min(m + K, T);
+        hip_check(hipMemcpyAsync(seg, cp + (2 * j) * npts, 2 * npts * sizeof(real_t),
+                                 hipMemcpyDeviceToDevice, st), "restore checkpoint");
+        std::vector<StepSlots> sp;
+        for (size_t n = n_lo; n <= n_hi; n++)
+            sp.push_back(StepSlots{n - 1 - m, n - m, n + 1 - m, false, 0, 0});
+        seg_args.n0 = n_lo;
+        seg_args.n1 = n_hi;
+        auto seg_ptr = [&](size_t l) { return seg + l * npts; };
+        time_loop(seg_args, pr, sp, 0, sp.size(), nullptr, 0, 0u, st, seg_ptr, nop, nop,
+                  noswap, no_uploads);
+        for (size_t n = n_hi; n >= n_lo; n--) {
+            const size_t k = n + 1;
+            real_t *ub = ubar + (k % 3) * npts;
+            real_t *ub1 = ubar + ((k - 1) % 3) * npts;
+            real_t *ub2 = ubar + ((k - 2) % 3) * npts;
+            if (k <= T && a.nrec)
+                hipLaunchKernelGGL(sw::k_adj_receivers, dim3((unsigned)a.nrec), dim3(256), 0,
+                                   st, P, pr.rbox, pr.rw, (int)a.nrec,
+                                   rec_adj + (k - 1) * a.nrec, ub);
+#if SW_DIM == 3
+            const int axes[3] = {0, 1, 2};
+            const long long lines[3] = {
+                (long long)(P.nx - 2 * r) * (P.ny - 2 * r),
+                (long long)(P.nz - 2 * r) * (P.ny - 2 * r),
+                (long long)(P.nz - 2 * r) * (P.nx - 2 * r)};
+            const int nph = 3;
+#else
+            const int axes[2] = {0, 2};
+            const long long lines[2] = {(long long)(P.nx - 2 * r), (long long)(P.nz - 2 * r)};
+            const int nph = 2;
+#endif
+            for (int ph = 0; ph < nph; ph++)   // z first: the reverse of y, x, z
+                hipLaunchKernelGGL(sw::k_adj_bc_phase,
+                                   dim3((unsigned)((lines[ph] + 255) / 256)), dim3(256), 0, st,
+                                   P, ub, axes[ph]);
+            hipLaunchKernelGGL(sw::k_adj_cw, dim3(eb), dim3(256), 0, st, P, ub, W,
+                               (long long)npts);
+            if (a.nsrc && (gw || !image))
+                hipLaunchKernelGGL(sw::k_adj_sources, dim3((unsigned)a.nsrc), dim3(256), 0, st,
+                                   P, a.vel, ub, (int)n, image ? nullptr : grad,
+                                   gw ? gw + (n - 1) * a.nsrc : nullptr);
+            hipLaunchKernelGGL(sw::k_adj_update, dim3(eb), dim3(256), 0, st, P, a.vel, ub, W,
+                               ub1, ub2, seg + (n - m) * npts, seg + (n - 1 - m) * npts,
+                               image ? seg + (n + 1 - m) * npts : nullptr, grad,
+                               (long long)npts);
+            zero(ub, npts);
+            hip_check(hipGetLastError(), "adjoint step");
+            if (n == n_lo)
+                break;
+        }
+    }
+    hip_check(hipStreamSynchronize(st), "adjoint");
+    g_timings[1] = t_fwd - t0;
+    g_timings[2] = now_s() - t_fwd;
+    const double total = now_s() - t0;
+    g_timings[0] = total;
+    return total;
+#endif
+}
+
 template <typename F>
 double guarded(F &&f)
 {
@@ -1627,6 +1788,18 @@ SW_EXPORT int SW_SUFFIX(simwave_hip_plan_destroy_)(void *plan)
         return 0.0;
     });
     return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT double SW_SUFFIX(simwave_hip_gradient_)(SW_FORWARD_PARAMS, const real_t *rec_adjoint,
+                                                 double *grad_velocity,
+                                                 real_t *grad_wavelet,
+                                                 size_t checkpoint_every, void *stream,
+                                                 unsigned flags)
+{
+    return guarded([&] {
+        return run_adjoint(fill_args(SW_FORWARD_ARGS), rec_adjoint, grad_velocity,
+                           grad_wavelet, checkpoint_every, (hipStream_t)stream, flags);
+    });
 }
 
 SW_EXPORT const char *simwave_hip_last_error(void)

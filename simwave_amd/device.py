@@ -45,7 +45,9 @@ class DeviceForward:
     device : str or torch.device, optional
     """
 
-    def __init__(self, solver, device="cuda", end_timestep=None):
+    def __init__(self, solver, device="cuda", end_timestep=None, velocity=None):
+        """velocity: optional device tensor of the extended shape used in
+        place of the solver's extended velocity model."""
         import torch
         _TORCH_DTYPE.update(float32=torch.float32, float64=torch.float64)
 
@@ -66,6 +68,13 @@ class DeviceForward:
         for key in _DEVICE_KEYS:
             host = kwargs.get(key)
             if host is None:
+                continue
+            if key == "velocity_model" and velocity is not None:
+                if tuple(velocity.shape) != host.shape or \
+                        velocity.dtype != _TORCH_DTYPE[str(host.dtype)]:
+                    raise ValueError("velocity tensor must have shape %s and the model dtype"
+                                     % (host.shape,))
+                self.tensors[key] = velocity.contiguous()
                 continue
             if key in ("u_full", "shot_record") and not host.any():
                 # the Solver's fresh slots and record are zeros: no upload
@@ -94,6 +103,7 @@ class DeviceForward:
         self._plan_destroy.restype = ctypes.c_int
         self._plan_destroy.argtypes = [ctypes.c_void_p]
         self._plan = None
+        self._argtypes = argtypes
 
     @property
     def device_bytes(self):
