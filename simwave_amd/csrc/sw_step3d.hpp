@@ -25,6 +25,8 @@
 // order 1 the columns are cut into 8 contiguous chunks, one per XCD (blocks
 // are dispatched to XCDs round-robin), so tiles sharing halo rows share an L2.
 #pragma once
+#include <type_traits>
+
 #include "sw_kernels.hpp"
 
 namespace sw {
@@ -107,6 +109,11 @@ __device__ __forceinline__ unsigned long long damp_bits(const Tiling &T, int col
 }
 
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+// A buffer offset past the end of every descriptor the tiled kernel builds
+// (their ranges stay below 2^31 bytes): loads return zeros, stores are
+// dropped.
+constexpr unsigned kOOB = 0x80000000u;
 
 __device__ __forceinline__ rsrc_t make_rsrc(const void *base, unsigned long long bytes)
 {
@@ -237,9 +244,11 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     const int z1 = min(z0 + T.zlen, nz - R);
     const int zb = z0 - R;                  // first plane the tile touches
 
-    // descriptors rebased at plane zb
+    // descriptors rebased at plane zb, covering the planes the tile reads,
+    // zb .. z1+R-1 (at most 64+2R planes: below 2^31 bytes, checked by the
+    // host), so that kOOB is out of range for every descriptor
     const unsigned long long off0 = (unsigned long long)zb * PB;
-    const unsigned long long rem = (unsigned long long)(nz - zb) * PB;
+    const unsigned long long rem = (unsigned long long)(z1 + R - zb) * PB;
     const rsrc_t r_uc = make_rsrc(A.uc + off0 / esz, rem);
     const rsrc_t r_up = make_rsrc(A.up + off0 / esz, rem);
     const rsrc_t r_q = make_rsrc(A.qc + off0 / esz, rem);
@@ -310,8 +319,23 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
             smask |= 1ull << s;
     }
 
+    // A clean tile touches no boundary face and no source box, and each of
+    // its lanes holds a vector that is wholly interior or wholly outside.
+    // Its march issues the same memory instructions on every plane (loads
+    // that are not needed and stores of lanes outside go to kOOB, which the
+    // descriptor range check turns into zeros / dropped writes), so the
+    // compiler's wait counts only ever wait for the data a plane consumes.
+    const bool lanes_whole = ((ny - 2 * R) % V == 0) || (k0 + TY <= ny - R);
+    const bool clean = !bc_xy && smask == 0 && !(bz_b && z0 <= r2) &&
+                       !(bz_a && z1 - 1 >= nz - r2 - 1) && lanes_whole;
+
+    // The march is instantiated twice, for clean and other tiles, with
+    // everything it keeps in registers declared inside, so the two share
+    // no live values.
+    auto march = [&](auto clean_tag) {
+    constexpr bool CLEAN = decltype(clean_tag)::value;
     // prologue: z queue planes zb .. zb+2R-1 (plane offset in the VGPR
-    // offset: range checked against the array end)
+    // offset: range checked against the descriptor end)
     vreal qz[RPT][2 * R + 1];
 #pragma unroll
     for (int m = 0; m < 2 * R; m++)
@@ -350,9 +374,47 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                 ne[q] = vload(r_e, cb[q], sp);
         }
     };
-    load_centre(z0);
-    load_update(z0);
+    // Clean-tile loads: always issued; the prefetch past the last plane
+    // and the damping term of undamped planes read kOOB (no traffic).
+    auto load_centre_clean = [&](int I) {
+        const bool live = I < z1;
+        const unsigned sp = live ? (unsigned)(I - zb) * PB : 0u;
+        const unsigned vc = (unsigned)(I + R - zb) * PB;
+#pragma unroll
+        for (int q = 0; q < RPT; q++)
+            nc[q] = vload(r_uc, live ? cb[q] + vc : kOOB, 0u);
+#pragma unroll
+        for (int h = 0; h < HX; h++)
+            nhx[h] = vload(r_uc, live ? xh_off[h] : kOOB, sp);
+#pragma unroll
+        for (int h = 0; h < HY; h++)
+            nhy[h] = vload(r_uc, live ? yh_off[h] : kOOB, sp);
+    };
+    auto load_update_clean = [&](int I) {
+        const bool live = I < z1;
+        const bool damp = live && ((dbits >> (I - z0)) & 1);
+        const unsigned sp = live ? (unsigned)(I - zb) * PB : 0u;
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            np[q] = vload(r_up, live ? cb[q] : kOOB, sp);
+            nq[q] = vload(r_q, live ? cb[q] : kOOB, sp);
+            ne[q] = vload(r_e, damp ? cb[q] : kOOB, sp);
+        }
+    };
+    if constexpr (CLEAN) {
+        load_centre_clean(z0);
+        load_update_clean(z0);
+    } else {
+        load_centre(z0);
+        load_update(z0);
+    }
 
+    // store offsets of clean tiles: rows past the interior and lanes
+    // outside it write nothing
+    unsigned st_off[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++)
+        st_off[q] = (act[q] && full) ? cb[q] : kOOB;
     for (int i = z0; i < z1; i++) {
         real_t *lb = &lds[(i - z0) & 1][0][0];
         const bool damped = (dbits >> (i - z0)) & 1;
@@ -373,12 +435,15 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
             if (yh_lds[h] >= 0)
                 *(vreal *)&lb[yh_lds[h]] = nhy[h];
         const bool more = i + 1 < z1;
-        if (more)
+        if constexpr (CLEAN)
+            load_centre_clean(i + 1);
+        else if (more)
             load_centre(i + 1);
         __syncthreads();
 
-        const bool bc_here = bc_xy || (bz_b && i <= r2) || (bz_a && i >= nz - r2 - 1);
-        const bool special = (smask != 0) | bc_here;
+        const bool bc_here =
+            !CLEAN && (bc_xy || (bz_b && i <= r2) || (bz_a && i >= nz - r2 - 1));
+        const bool special = !CLEAN && ((smask != 0) | bc_here);
         const unsigned so = (unsigned)(i - zb) * PB;
         vreal out[RPT];
         // x neighbours of the lane's RPT adjacent rows: rows -R .. RPT-1+R
@@ -469,7 +534,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                                                      np[q][e])
                                    : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
             }
-            if (special && act[q]) {
+            if (!CLEAN && special && act[q]) {
                 // tiles touching a source box or a boundary face
                 const int j = j0 + slot * RPT + q;
 #pragma unroll
@@ -489,24 +554,38 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                 }
             }
         }
-        if (more)
-            load_update(i + 1);
-        if (!special) {
+        // stores first, then the next plane's update operands, so that
+        // waiting for those never waits for the stores
+        if constexpr (CLEAN) {
 #pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                if (!act[q])
-                    continue;
-                if (full) {
-                    vstore(out[q], r_un, cb[q], so);
-                } else {
+            for (int q = 0; q < RPT; q++)
+                vstore(out[q], r_un, st_off[q], so);
+            load_update_clean(i + 1);
+        } else {
+            if (!special) {
 #pragma unroll
-                    for (int e = 0; e < V; e++)
-                        if (k + e < ny - R)
-                            sstore(out[q][e], r_un, cb[q] + e * esz, so);
+                for (int q = 0; q < RPT; q++) {
+                    if (!act[q])
+                        continue;
+                    if (full) {
+                        vstore(out[q], r_un, cb[q], so);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < V; e++)
+                            if (k + e < ny - R)
+                                sstore(out[q][e], r_un, cb[q] + e * esz, so);
+                    }
                 }
             }
+            if (more)
+                load_update(i + 1);
         }
     }
+    };
+    if (clean)
+        march(std::true_type{});
+    else
+        march(std::false_type{});
 }
 
 #endif  // SW_DIM == 3

@@ -80,6 +80,18 @@ CASES = {
                        tf=0.05),
     "3d_wide_so6": dict(dim=3, shape=(24, 26, 270), space_order=6, damping=30.0,
                         tf=0.05),
+    # deep grids: with 8-plane z chunks most tiles touch no boundary face and
+    # no source box (the tiled kernel's branch-free march)
+    "3d_deep_so8": dict(dim=3, shape=(60, 30, 301), space_order=8, damping=30.0,
+                        tf=0.04),
+    "3d_deep_so8_whole": dict(dim=3, shape=(58, 40, 264), space_order=8, damping=20.0,
+                              tf=0.04),
+    "3d_deep_none": dict(dim=3, shape=(56, 34, 200), space_order=8, damping=0.0,
+                         bc=("none",) * 6, src=[(300.0, 150.0, 1000.0)], tf=0.05),
+    "3d_deep_f64": dict(dim=3, shape=(50, 28, 140), space_order=8, damping=30.0,
+                        dtype=F64, tf=0.03),
+    "3d_deep_so4": dict(dim=3, shape=(52, 30, 260), space_order=4, damping=30.0,
+                        tf=0.04),
 }
 
 
@@ -133,6 +145,32 @@ def _tiled(name, tile, order):
     finally:
         os.environ.pop("SIMWAVE_HIP_TILE", None)
         os.environ.pop("SIMWAVE_HIP_ORDER", None)
+
+
+DEEP_R4 = ["3d_deep_so8", "3d_deep_so8_whole", "3d_deep_none", "3d_deep_f64"]
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("tile", list(range(14)))
+@pytest.mark.parametrize("name", DEEP_R4)
+def test_parity_clean_tiles_r4(gpu, name, tile, order):
+    """Tiles away from every face and source (8-plane z chunks), every
+    radius-4 configuration."""
+    os.environ["SIMWAVE_HIP_ZLEN"] = "8"
+    try:
+        _tiled(name, tile, order)
+    finally:
+        os.environ.pop("SIMWAVE_HIP_ZLEN", None)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("tile", [0, 4, 8])
+def test_parity_clean_tiles_r2(gpu, tile, order):
+    os.environ["SIMWAVE_HIP_ZLEN"] = "6"
+    try:
+        _tiled("3d_deep_so4", tile, order)
+    finally:
+        os.environ.pop("SIMWAVE_HIP_ZLEN", None)
 
 
 @pytest.mark.parametrize("order", [0, 1])
