@@ -128,6 +128,96 @@ def faulted_2d_vd(dtype=np.float32, seed=0):
                 wavelet=sw.RickerWavelet(25.0, tm))
 
 
+def fwi_3d(dtype=np.float32, seed=0, shot=0):
+    """FWI shot: 160 x 256 x 256 at 15 m, space order 8, 30-cell cubic
+    damping (228 x 324 x 324 extended), 5 flat layers 1500..4000 m/s times
+    (1 + p), p seeded smooth noise (max |p| 5%); tf 1.2 s; source `shot` of
+    16 on a 4 x 4 grid at 30 m depth; 100 x 100 receivers at 15 m depth."""
+    import simwave_amd as sw
+    from scipy.ndimage import gaussian_filter
+
+    nz, nx, h = 160, 256, 15.0
+    rng = np.random.default_rng(seed)
+    noise = gaussian_filter(rng.standard_normal((nz, nx, nx), dtype=np.float32),
+                            6.0, mode="reflect", truncate=3.0)
+    noise *= np.float32(0.05 / np.abs(noise).max())
+    layers = np.linspace(1500.0, 4000.0, 5)
+    vel = np.repeat(layers, nz // 5).astype(np.float32)[:, None, None] * (1 + noise)
+    vel = vel.astype(dtype)
+    ez, ex = (nz - 1) * h, (nx - 1) * h
+    sm = sw.SpaceModel(bounding_box=(0, ez, 0, ex, 0, ex), grid_spacing=(h, h, h),
+                       velocity_model=vel, space_order=8, dtype=dtype)
+    sm.config_boundary(damping_length=30 * h,
+                       boundary_condition=("null_neumann",) + ("null_dirichlet",) * 5,
+                       damping_polynomial_degree=3, damping_alpha=0.001)
+    tm = sw.TimeModel(space_model=sm, tf=1.2, saving_stride=0)
+    sx = ex * (0.5 + (shot % 4)) / 4
+    sy = ex * (0.5 + (shot // 4 % 4)) / 4
+    src = sw.Source(sm, coordinates=[(30.0, sx, sy)])
+    g = np.linspace(0.0, ex, 100)
+    rec = sw.Receiver(sm, coordinates=[(15.0, x, y) for x in g for y in g])
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(10.0, tm))
+
+
+def run_fwi(args, desc):
+    """Adjoint-state gradient of one FWI shot, device resident: forward with
+    checkpoints every 20 steps, then the reverse sweep with recomputation.
+    value = the shot's interior updates / gradient seconds."""
+    import torch
+    import simwave_amd as sw
+    from simwave_amd import adjoint
+    from simwave_amd.device import DeviceForward
+
+    t0 = time.time()
+    solver = sw.Solver(**fwi_3d())
+    dev = DeviceForward(solver)
+    kw = dev.kwargs
+    T, nrec = kw["end_timestep"], kw["num_receivers"]
+    res = torch.from_numpy(np.random.default_rng(args.seed).standard_normal(
+        (T, nrec)).astype(np.float32)).to(dev.device)
+    log("workload %s: grid %s, %d steps, %d receivers, host prep %.1fs"
+        % (args.workload, kw["velocity_model"].shape, T, nrec, time.time() - t0))
+    every = 20
+    for _ in range(args.warmup):
+        dev.reset()
+        adjoint._run(dev, res, every, False)
+    torch.cuda.synchronize()
+    grad_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        dev.reset()
+        adjoint._run(dev, res, every, False)     # synchronises before returning
+        grad_ms.append((time.perf_counter() - ts) * 1e3)
+        log("gradient %d done (%.1f ms)" % (i, grad_ms[-1]))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fwd_ms = []
+    for _ in range(2):
+        dev.reset()
+        ts = time.perf_counter()
+        dev.run()
+        fwd_ms.append((time.perf_counter() - ts) * 1e3)
+    upd = dev.updates_per_shot
+    out = {
+        "metric": METRIC, "value": round(upd / (elapsed / args.steps) / 1e9, 3),
+        "unit": "Gpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc,
+                   "grid": list(kw["velocity_model"].shape), "time_steps": T,
+                   "updates_per_shot": upd, "checkpoint_every": every,
+                   "receivers": nrec, "parallelism": "none"},
+        "gradient": {"ms_p50": round(float(np.percentile(grad_ms, 50)), 2),
+                     "forward_ms": round(min(fwd_ms), 2),
+                     "gradient_over_forward": round(float(np.percentile(grad_ms, 50))
+                                                    / min(fwd_ms), 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_host_workload(args, builder, desc):
     """Times Solver.forward() (the reference-compatible host entry: models
     and every returned snapshot cross PCIe inside the call)."""
@@ -309,6 +399,9 @@ WORKLOADS = {
     "2d_cd_so4_readme": (readme_2d,
                          "2D constant density so4 512^2 (README example)"),
     "serving": (None, "mixed shot stream through the continuous-batching server"),
+    "fwi_3d_so8_256": (fwi_3d, "adjoint-state velocity gradient of one FWI shot: 3D so8 "
+                               "160x256x256 (228x324x324 extended), 15 m, tf 1.2 s, "
+                               "100x100 receivers, checkpoints every 20 steps"),
     "2d_vd_so16_stride10": (faulted_2d_vd,
                             "2D variable density so16 1000x3000 (1136x3136 extended), "
                             "5 m, 60-pt damping, faulted layers, tf 3.0 s, 3000 "
@@ -431,6 +524,9 @@ def main():
         sys.exit(1)
     if args.workload == "serving":
         run_serving(args)
+        return
+    if args.workload == "fwi_3d_so8_256":
+        run_fwi(args, WORKLOADS[args.workload][1])
         return
     if args.workload == "2d_vd_so16_stride10":
         builder, desc = WORKLOADS[args.workload]

@@ -118,8 +118,9 @@ int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -222,8 +223,9 @@ int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -326,8 +328,9 @@ int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -430,8 +433,9 @@ int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_vd_f64(
     double *u, double *velocity, double *density, double *damp,
@@ -534,8 +538,9 @@ int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -638,8 +643,9 @@ int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -742,8 +748,9 @@ int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -846,8 +853,9 @@ int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
  * forward states saved every that many steps and recomputed per segment.
- * flags: 0, or 1 to set grad_velocity to the zero-lag cross-correlation
- * image sum_k u^k lambda^k of the forward and adjoint wavefields instead.
+ * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
+ * the forward's receiver gather on a second stream (large grids).
  * Also writes the shot record.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_vd_f64(
     double *u, double *velocity, double *density, double *damp,

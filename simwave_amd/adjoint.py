@@ -24,6 +24,7 @@ from .middleware import _last_error
 __all__ = ["gradient", "misfit_gradient", "image", "ShotRecord"]
 
 IMAGE = 1     # adjoint entry flag: cross-correlation image instead of gradient
+OVERLAP_RECEIVERS = 4   # forward receiver gather on a second stream (large grids)
 
 
 def _check(solver):
@@ -33,7 +34,8 @@ def _check(solver):
         raise ValueError("the adjoint needs saving_stride 0")
 
 
-def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=False):
+def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=False,
+         overlap_receivers=True):
     import torch
     kw = dev.kwargs
     lib = dev._lib
@@ -56,7 +58,8 @@ def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=Fa
     secs = fn(*dev._args, ctypes.c_void_p(rec.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
               ctypes.c_void_p(gw.data_ptr() if gw is not None else 0),
               ctypes.c_size_t(int(checkpoint_every)), ctypes.c_void_p(stream.cuda_stream),
-              ctypes.c_uint(IMAGE if image else 0))
+              ctypes.c_uint((IMAGE if image else 0)
+                            | (OVERLAP_RECEIVERS if overlap_receivers else 0)))
     if secs < 0:
         raise RuntimeError("adjoint failed: {}".format(_last_error(lib)))
     return grad, gw

@@ -53,18 +53,36 @@ __device__ __forceinline__ real_t stencil_clipped(const Params &P, const real_t 
 #endif
 }
 
-__device__ __forceinline__ void decode(const Params &P, long long p, int &i, int &j, int &k)
+// Point of this thread in a whole-grid launch (adj_grid): 64 threads along
+// the contiguous axis, 4 lines per block.
+constexpr int kAdjLines = 4;
+
+__device__ __forceinline__ bool thread_point(const Params &P, int &i, int &j, int &k)
 {
-    i = (int)(p / P.sz);
-    const long long rem = p - (long long)i * P.sz;
 #if SW_DIM == 3
-    j = (int)(rem / P.sx);
-    k = (int)(rem - (long long)j * P.sx);
+    i = blockIdx.z;
+    j = blockIdx.y * kAdjLines + threadIdx.y;
+    k = blockIdx.x * 64 + threadIdx.x;
+    return j < P.nx && k < P.ny;
 #else
-    j = (int)rem;
+    i = blockIdx.y * kAdjLines + threadIdx.y;
+    j = blockIdx.x * 64 + threadIdx.x;
     k = 0;
+    return i < P.nz && j < P.nx;
 #endif
 }
+
+inline dim3 adj_grid(const Params &P)
+{
+#if SW_DIM == 3
+    return dim3((unsigned)((P.ny + 63) / 64), (unsigned)((P.nx + kAdjLines - 1) / kAdjLines),
+                (unsigned)P.nz);
+#else
+    return dim3((unsigned)((P.nx + 63) / 64), (unsigned)((P.nz + kAdjLines - 1) / kAdjLines), 1u);
+#endif
+}
+
+inline dim3 adj_block() { return dim3(64u, (unsigned)kAdjLines, 1u); }
 
 __device__ __forceinline__ bool interior(const Params &P, int i, int j, int k)
 {
@@ -169,13 +187,12 @@ __global__ void k_adj_bc_phase(Params P, real_t *ub, int axis)
 }
 
 // W = c * ubar on the interior, 0 elsewhere.
-__global__ void k_adj_cw(Params P, const real_t *__restrict__ ub, real_t *__restrict__ W,
-                         long long npts)
+__global__ void __launch_bounds__(64 * kAdjLines)
+k_adj_cw(Params P, const real_t *__restrict__ ub, real_t *__restrict__ W)
 {
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
-         p += (long long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        decode(P, p, i, j, k);
+    int i, j, k;
+    if (thread_point(P, i, j, k)) {
+        const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
         real_t w = 0;
         if (interior(P, i, j, k)) {
             const double den = 1.0 + (double)P.e[p];
@@ -186,58 +203,64 @@ __global__ void k_adj_cw(Params P, const real_t *__restrict__ ub, real_t *__rest
 }
 
 // The transpose of the interior update of step n (k = n+1) and the
-// velocity derivative at every point:
-//   ub1 += a w + L^T W,  ub2 -= b w  (w = ub, the adjoint of u^{n+1}),
-//   grad += w (da (u1 - u2) + dc L u1)  with u1 = u^n, u2 = u^{n-1}.
-// With `u0` set (image mode) grad collects the zero-lag cross-correlation
-// w u^{n+1} (u0 = u^{n+1}) instead.
-__global__ void k_adj_update(Params P, const real_t *__restrict__ vel,
-                             const real_t *__restrict__ ub, const real_t *__restrict__ W,
-                             real_t *__restrict__ ub1, real_t *__restrict__ ub2,
-                             const real_t *__restrict__ u1, const real_t *__restrict__ u2,
-                             const real_t *__restrict__ u0, double *__restrict__ grad,
-                             long long npts)
+// velocity derivative at every point (w = ub, the adjoint of u^{n+1};
+// u0, u1, u2 = u^{n+1}, u^n, u^{n-1}):
+//   ub1 += a w + L^T W,  ub2 -= b w,
+//   grad += w (da (u1 - u2) + dc (L u1 + s))
+// where s is the source term of the step.  c (L u1 + s) is what the step
+// added to a u1 - b u2, so (L u1 + s) = (u0 - a u1 + b u2) / c and
+// dc / c = 2 / (v den): no second stencil is needed.  Points the boundary
+// phases zeroed (Dirichlet lines) have w = 0.  In image mode grad collects
+// the zero-lag cross-correlation w u0 instead.  ub is cleared on the way
+// (its buffer becomes the adjoint of u^{n-2}).
+__global__ void __launch_bounds__(64 * kAdjLines)
+k_adj_update(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
+             const real_t *__restrict__ W, real_t *__restrict__ ub1, real_t *__restrict__ ub2,
+             const real_t *__restrict__ u0, const real_t *__restrict__ u1,
+             const real_t *__restrict__ u2, double *__restrict__ grad, int image)
 {
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
-         p += (long long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        decode(P, p, i, j, k);
+    int i, j, k;
+    if (thread_point(P, i, j, k)) {
+        const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
         const real_t lt = stencil_clipped(P, W, i, j, k);
         if (!interior(P, i, j, k)) {
             ub1[p] += lt;
-            continue;
+            ub[p] = 0;
+            return;
         }
         const real_t w = ub[p];
-        const double e = (double)P.e[p], q = (double)P.q[p], v = (double)vel[p];
-        const double den = 1.0 + e;
-        const double a = 2.0 / den, b = (1.0 - e) / den;
+        ub[p] = 0;
+        const double e = (double)P.e[p];
+        const double rden = 1.0 / (1.0 + e);
+        const double a = 2.0 * rden, b = (1.0 - e) * rden;
         ub1[p] = (real_t)((double)ub1[p] + a * (double)w + (double)lt);
         ub2[p] = (real_t)((double)ub2[p] - b * (double)w);
-        if (u0) {
-            grad[p] += (double)w * (double)u0[p];
-        } else if (w != (real_t)0) {
-            const double da = -4.0 * e / (v * den * den);
-            const double dc = 2.0 * q / (v * den * den);
-            const double lu = (double)stencil_clipped(P, u1, i, j, k);
-            grad[p] += (double)w * (da * ((double)u1[p] - (double)u2[p]) + dc * lu);
+        if (w != (real_t)0) {
+            const double x0 = (double)u0[p], x1 = (double)u1[p], x2 = (double)u2[p];
+            if (image) {
+                grad[p] += (double)w * x0;
+            } else {
+                const double rv = 1.0 / (double)vel[p];
+                const double da = -4.0 * e * rden * rden * rv;
+                const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
+                grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+            }
         }
     }
 }
 
-// Source terms of step n (k = n+1): the velocity derivative of
-// c kws w^{n-1} (skipped when `grad` is null) and the wavelet gradient
-// sum_p kws c_p ub_p.  One block per source; the wavelet gradient of source
-// s at step n goes to gw[s].
+// Wavelet gradient of step n (k = n+1): gw[s] = sum_p kws_{s,p} c_p ub_p
+// (c = q / den) over the source box's interior points, in float64, one
+// block per source.  (The velocity derivative of the source term is part
+// of k_adj_update's step difference; `grad` is unused and may be null.)
 __global__ void __launch_bounds__(256)
-k_adj_sources(Params P, const real_t *__restrict__ vel, const real_t *__restrict__ ub,
-              int n, double *__restrict__ grad, real_t *__restrict__ gw)
+k_adj_sources(Params P, const real_t *__restrict__ ub, real_t *__restrict__ gw)
 {
     __shared__ double part[256];
     const int s = blockIdx.x;
     if (s >= P.nsrc)
         return;
     const Box b = P.sbox[s];
-    const real_t amp = wavelet_amp(P, n, s);
     const int cz = b.hi[0] - b.lo[0] + 1, cx = b.hi[1] - b.lo[1] + 1;
 #if SW_DIM == 3
     const int cy = b.hi[2] - b.lo[2] + 1;
@@ -259,12 +282,8 @@ k_adj_sources(Params P, const real_t *__restrict__ vel, const real_t *__restrict
             continue;
         const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
         const double kws = (double)box_weight(b, P.sw, i, j, k);
-        const double e = (double)P.e[p], q = (double)P.q[p], v = (double)vel[p];
-        const double den = 1.0 + e;
-        const double w = (double)ub[p];
-        acc += kws * (q / den) * w;
-        if (grad && amp != (real_t)0 && w != 0.0)
-            atomicAdd(&grad[p], w * (2.0 * q / (v * den * den)) * kws * (double)amp);
+        const double q = (double)P.q[p], den = 1.0 + (double)P.e[p];
+        acc += kws * (q / den) * (double)ub[p];
     }
     part[threadIdx.x] = acc;
     __syncthreads();
@@ -273,7 +292,7 @@ k_adj_sources(Params P, const real_t *__restrict__ vel, const real_t *__restrict
             part[threadIdx.x] += part[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && gw)
+    if (threadIdx.x == 0)
         gw[s] = (real_t)part[0];
 }
 

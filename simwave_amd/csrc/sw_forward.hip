@@ -1510,8 +1510,8 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
         fail("NULL gradient array");
     if (a.nrec && !rec_adj)
         fail("NULL shot-record adjoint");
-    if (flags & ~kAdjImage)
-        fail("unknown adjoint flags 0x%x", flags & ~kAdjImage);
+    if (flags & ~(kAdjImage | kFlagOverlapReceivers))
+        fail("unknown adjoint flags 0x%x", flags & ~(kAdjImage | kFlagOverlapReceivers));
     const bool image = flags & kAdjImage;
     const int r = (int)(a.space_order / 2);
     const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : (size_t)(3 * r + 1)};
@@ -1562,14 +1562,13 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     auto nop = [](size_t) {};
     auto noswap = [](size_t, size_t) {};
     auto no_uploads = [](size_t) { return false; };
-    time_loop(a, pr, plan, 0, plan.size(), a.receivers, 0, 0u, st, slot_ptr, save, nop, noswap,
-              no_uploads);
+    time_loop(a, pr, plan, 0, plan.size(), a.receivers, 0, flags & kFlagOverlapReceivers, st,
+              slot_ptr, save, nop, noswap, no_uploads);
     const double t_fwd = now_s();
 
     // backward over the checkpoint segments
     Args seg_args = a;
     seg_args.nrec = 0;
-    const unsigned eb = (unsigned)std::min<size_t>((npts + 255) / 256, 16384);
     for (size_t j = ncp; j-- > 0;) {
         const size_t m = j * K;                   // levels m, m+1 saved
         const size_t n_lo = m + 1, n_hi = std::min(m + K, T);
@@ -1608,17 +1607,14 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
                 hipLaunchKernelGGL(sw::k_adj_bc_phase,
                                    dim3((unsigned)((lines[ph] + 255) / 256)), dim3(256), 0, st,
                                    P, ub, axes[ph]);
-            hipLaunchKernelGGL(sw::k_adj_cw, dim3(eb), dim3(256), 0, st, P, ub, W,
-                               (long long)npts);
-            if (a.nsrc && (gw || !image))
+            hipLaunchKernelGGL(sw::k_adj_cw, sw::adj_grid(P), sw::adj_block(), 0, st, P, ub, W);
+            if (a.nsrc && gw)
                 hipLaunchKernelGGL(sw::k_adj_sources, dim3((unsigned)a.nsrc), dim3(256), 0, st,
-                                   P, a.vel, ub, (int)n, image ? nullptr : grad,
-                                   gw ? gw + (n - 1) * a.nsrc : nullptr);
-            hipLaunchKernelGGL(sw::k_adj_update, dim3(eb), dim3(256), 0, st, P, a.vel, ub, W,
-                               ub1, ub2, seg + (n - m) * npts, seg + (n - 1 - m) * npts,
-                               image ? seg + (n + 1 - m) * npts : nullptr, grad,
-                               (long long)npts);
-            zero(ub, npts);
+                                   P, ub, gw + (n - 1) * a.nsrc);
+            hipLaunchKernelGGL(sw::k_adj_update, sw::adj_grid(P), sw::adj_block(), 0, st, P,
+                               a.vel, ub, W, ub1, ub2, seg + (n + 1 - m) * npts,
+                               seg + (n - m) * npts, seg + (n - 1 - m) * npts, grad,
+                               image ? 1 : 0);
             hip_check(hipGetLastError(), "adjoint step");
             if (n == n_lo)
                 break;
