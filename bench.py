@@ -385,6 +385,9 @@ def run_serving(args):
         "serving": {"updates": summ["updates"], "makespan_s": round(summ["makespan_s"], 3),
                     "queue_p50_ms": round(summ["queue_p50_s"] * 1e3, 2),
                     "max_active_seen": summ["max_active"],
+                    "prep_mean_ms": round(summ["prep_mean_s"] * 1e3, 2),
+                    "device_mean_ms": round(summ["device_mean_s"] * 1e3, 2),
+                    "fetch_mean_ms": round(summ["fetch_mean_s"] * 1e3, 2),
                     "plain_path_gpts": round(plain, 3),
                     "speedup_vs_plain": round(summ["gpts"] / plain, 3)},
         "roofline": None, "cpu_baseline": None,
@@ -592,7 +595,9 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = time_cpu(kw, dev.variant, args.cpu_seconds)
+        from simwave_amd.middleware import Middleware
+        cpu = time_cpu(Middleware.normalize(**solver.forward_arguments()), dev.variant,
+                       args.cpu_seconds)
         if cpu:
             log("cpu baseline: %s" % cpu)
 

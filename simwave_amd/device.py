@@ -53,7 +53,9 @@ class DeviceForward:
 
         self._torch = torch
         self.solver = solver
-        kwargs = Middleware.normalize(**solver.forward_arguments())
+        # the big arrays (slots, record, extended models, damping mask) are
+        # built on the device; the host keeps placeholders of their shape
+        kwargs = Middleware.normalize(**solver._arguments(light=True))
         if end_timestep is not None:
             kwargs["end_timestep"] = min(int(end_timestep), kwargs["end_timestep"])
         self.variant = _variant(kwargs)
@@ -65,6 +67,7 @@ class DeviceForward:
         self._fn = getattr(lib, "simwave_hip_forward_device_" + self.variant)
         self.device = torch.device(device)
         self.tensors = {}
+        sm = solver.space_model
         for key in _DEVICE_KEYS:
             host = kwargs.get(key)
             if host is None:
@@ -75,14 +78,19 @@ class DeviceForward:
                     raise ValueError("velocity tensor must have shape %s and the model dtype"
                                      % (host.shape,))
                 self.tensors[key] = velocity.contiguous()
-                continue
-            if key in ("u_full", "shot_record") and not host.any():
-                # the Solver's fresh slots and record are zeros: no upload
+            elif key in ("u_full", "shot_record"):
+                # the Solver's fresh slots and record are zeros
                 self.tensors[key] = torch.zeros(host.shape, device=self.device,
                                                 dtype=_TORCH_DTYPE[str(host.dtype)])
-                continue
-            self.tensors[key] = torch.from_numpy(np.ascontiguousarray(host)).to(
-                self.device)
+            elif key == "damping_mask":
+                small, index = sm.damping_mask_factors()
+                self.tensors[key] = self._gather(small, index)
+            else:
+                model = sm.velocity_model if key == "velocity_model" else sm.density_model
+                self.tensors[key] = self._gather(model, sm.extension_index())
+            if tuple(self.tensors[key].shape) != host.shape:
+                raise RuntimeError("device %s has shape %s, expected %s"
+                                   % (key, tuple(self.tensors[key].shape), host.shape))
         self.kwargs = kwargs
         argtypes, args = Middleware.ordered(kwargs)
         keys = [k for k in Middleware._keys_in_order_static() if kwargs.get(k) is not None]
@@ -104,6 +112,16 @@ class DeviceForward:
         self._plan_destroy.argtypes = [ctypes.c_void_p]
         self._plan = None
         self._argtypes = argtypes
+
+    def _gather(self, small, index):
+        """small[np.ix_(*index)] built on the device: one upload of `small`,
+        then an index_select per axis (copies only, so the values are the
+        host's, bit for bit)."""
+        torch = self._torch
+        t = torch.from_numpy(np.ascontiguousarray(small)).to(self.device)
+        for axis, idx in enumerate(index):
+            t = torch.index_select(t, axis, torch.from_numpy(idx).to(self.device))
+        return t.contiguous()
 
     @property
     def device_bytes(self):
@@ -200,9 +218,23 @@ class DeviceForward:
         fn(out, 8)
         return list(out)
 
-    def results(self):
-        """(u_full, shot_record) as the Solver returns them."""
+    def results(self, stream=None):
+        """(u_full, shot_record) as the Solver returns them.  The returned
+        snapshots (one slot for saving_stride 0) are cut out of the slots on
+        the device, so only they cross PCIe, straight into pinned host
+        memory, on `stream` (default: torch's current stream), which is
+        synchronised before returning."""
+        torch = self._torch
         sm, tm = self.solver.space_model, self.solver.time_model
-        u = self.tensors["u_full"].cpu().numpy()
-        u = sm.remove_halo_region(tm.remove_time_halo_region(u))
-        return u, self.tensors["shot_record"].cpu().numpy()
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        with torch.cuda.stream(stream):
+            u = sm.remove_halo_region(tm.remove_time_halo_region(self.tensors["u_full"]))
+            u = u.contiguous()
+            rec = self.tensors["shot_record"]
+            u_h = torch.empty(u.shape, dtype=u.dtype, pin_memory=True)
+            r_h = torch.empty(rec.shape, dtype=rec.dtype, pin_memory=True)
+            u_h.copy_(u, non_blocking=True)
+            r_h.copy_(rec, non_blocking=True)
+        stream.synchronize()
+        return u_h.numpy(), r_h.numpy()

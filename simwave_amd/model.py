@@ -229,7 +229,11 @@ class SpaceModel:
         d is numpy's linear ramp from 0 at the inner edge to nbl at the outer
         edge, padded axis by axis; the halo is zero (model.py:377-406).
         """
-        mask = np.pad(np.zeros(self.shape, dtype=self.dtype),
+        return _pad_zero(self._damping_values(self.shape), self.halo_pad_width)
+
+    def _damping_values(self, core):
+        """The damping layer around a zero interior of shape `core`."""
+        mask = np.pad(np.zeros(core, dtype=self.dtype),
                       pad_width=self.nbl_pad_width, mode="linear_ramp",
                       end_values=self.nbl_pad_width)
         deg = self.damping_polynomial_degree
@@ -239,8 +243,34 @@ class SpaceModel:
             mask = np.power(mask, deg, out=np.zeros_like(mask), where=mask != 0)
         else:
             mask = mask ** deg
-        mask = mask * self.damping_alpha
-        return _pad_zero(mask, self.halo_pad_width)
+        return mask * self.damping_alpha
+
+    def damping_mask_factors(self):
+        """(small, index) with damping_mask == small[np.ix_(*index)].
+
+        np.pad's linear ramps depend only on the edge values and the pad
+        widths, never on the interior size, so the mask around a one-cell
+        interior holds every value of the full mask; `index` maps each
+        extended index to its cell of `small` (for building the mask where
+        the data lives, e.g. on a device, without the full host array)."""
+        small = _pad_zero(self._damping_values((1,) * self.dimension),
+                          self.halo_pad_width)
+        nbl, halo = self.nbl, self.halo_size
+        index = []
+        for a in range(self.dimension):
+            lo = halo[2 * a] + nbl[2 * a]
+            n = self.shape[a]
+            e = np.arange(self.extended_shape[a], dtype=np.int64)
+            index.append(np.where(e < lo, e, np.where(e < lo + n, lo, e - n + 1)))
+        return small, index
+
+    def extension_index(self):
+        """Per axis, the model index each extended index copies under the
+        edge padding of extended_velocity_model: clip(e - lo, 0, n - 1)."""
+        nbl, halo = self.nbl, self.halo_size
+        return [np.clip(np.arange(self.extended_shape[a], dtype=np.int64)
+                        - (halo[2 * a] + nbl[2 * a]), 0, self.shape[a] - 1)
+                for a in range(self.dimension)]
 
     @property
     def extended_grid(self):

@@ -81,6 +81,11 @@ class ServeStats:
             "latency_p99_s": percentile(lat, 99),
             "queue_p50_s": percentile([r["admitted"] - r["arrival"] for r in self.records], 50),
             "max_active": max(r["active_at_admission"] for r in self.records),
+            # mean per-request phases on its worker: argument preparation
+            # and uploads, enqueue-to-completion on the device, result fetch
+            "prep_mean_s": float(np.mean([r.get("prep", np.nan) for r in self.records])),
+            "device_mean_s": float(np.mean([r.get("device", np.nan) for r in self.records])),
+            "fetch_mean_s": float(np.mean([r.get("fetch", np.nan) for r in self.records])),
         }
 
 
@@ -216,18 +221,22 @@ class ShotServer:
                 ctx = torch.cuda.stream(stream)
             except Exception:                  # test doubles: no torch stream
                 ctx = _NullContext()
+            t0 = time.perf_counter()
             with ctx:
                 shot = self._shot_factory(req.solver)
             shot.plan(stream)
+            t1 = time.perf_counter()
             left = shot.steps
             while left > 0:
                 left = shot.run_steps(self.chunk_steps, stream)
             stream.synchronize()
-            result = shot.results()
+            t2 = time.perf_counter()
+            result = shot.results(stream)
             shot.release_plan()
             done = time.perf_counter()
             self.stats.add(arrival=req.arrival, admitted=req.admitted, done=done,
-                           updates=shot.updates_per_shot, active_at_admission=active_now)
+                           updates=shot.updates_per_shot, active_at_admission=active_now,
+                           prep=t1 - t0, device=t2 - t1, fetch=done - t2)
             req.future.set_result(result)
         except Exception as exc:
             if shot is not None:

@@ -81,17 +81,36 @@ class Solver:
     def forward_arguments(self):
         """Keyword arguments of the backend's forward, before marshalling
         (the argument set of solver.py:131-163)."""
+        return self._arguments(light=False)
+
+    def _arguments(self, light):
+        """forward_arguments(); with `light`, the wavefield slots, shot
+        record and extended models are zero-stride placeholders of the right
+        shape and dtype (no memory), for callers that build those arrays on
+        a device (simwave_amd.device.DeviceForward)."""
         sm, tm = self.space_model, self.time_model
         src_points, src_values, src_offsets = \
             self.sources.interpolated_points_and_values
         rec_points, rec_values, rec_offsets = \
             self.receivers.interpolated_points_and_values
-        u_full = self.u_full
+        if light:
+            def hollow(shape):
+                return np.broadcast_to(np.zeros((), dtype=sm.dtype), shape)
+            ext = sm.extended_shape
+            u_full = hollow((self.num_snapshots + 2,) + ext)
+            vel, damp = hollow(ext), hollow(ext)
+            den = None if sm.density_model is None else hollow(ext)
+            record = hollow((tm.timesteps, self.receivers.count))
+        else:
+            u_full = self.u_full
+            vel, den = sm.extended_velocity_model, sm.extended_density_model
+            damp = sm.damping_mask
+            record = self.shot_record
         return dict(
             u_full=u_full,
-            velocity_model=sm.extended_velocity_model,
-            density_model=sm.extended_density_model,
-            damping_mask=sm.damping_mask,
+            velocity_model=vel,
+            density_model=den,
+            damping_mask=damp,
             wavelet=self.wavelet.values,
             wavelet_size=self.wavelet.timesteps,
             wavelet_count=self.wavelet.num_sources,
@@ -108,7 +127,7 @@ class Solver:
             rec_points_values=rec_values,
             rec_points_values_offset=rec_offsets,
             rec_points_values_size=len(rec_values),
-            shot_record=self.shot_record,
+            shot_record=record,
             num_sources=self.sources.count,
             num_receivers=self.receivers.count,
             grid_spacing=sm.grid_spacing,

@@ -478,3 +478,49 @@ def test_padding_helpers_equal_numpy_pad():
                   tuple((i, len(shape) - i + 1) for i in range(len(shape)))]:
             assert np.array_equal(_pad_edge(a, w), np.pad(a, w, mode="edge"))
             assert np.array_equal(_pad_zero(a, w), np.pad(a, w))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("deg,alpha", [(3, 0.001), (2, 0.002), (1.5, 0.01), (4, 0.0005)])
+def test_damping_mask_factors_rebuild_the_mask(dim, dtype, deg, alpha):
+    """damping_mask == small[ix(index)] bit for bit, for uneven layers."""
+    rng = np.random.default_rng(dim * 7 + int(deg * 10))
+    for _ in range(3):
+        shape = tuple(int(rng.integers(5, 23)) for _ in range(dim))
+        h = 10.0
+        box = []
+        for n in shape:
+            box += [0.0, (n - 1) * h]
+        sm = sw.SpaceModel(bounding_box=tuple(box), grid_spacing=(h,) * dim,
+                           velocity_model=np.full(shape, 2000.0, dtype=dtype),
+                           space_order=int(rng.choice([2, 4, 8, 16])), dtype=dtype)
+        lengths = tuple(float(rng.integers(0, 9)) * h for _ in range(2 * dim))
+        sm.config_boundary(damping_length=lengths,
+                           boundary_condition=("null_neumann",) * (2 * dim),
+                           damping_polynomial_degree=deg, damping_alpha=alpha)
+        small, index = sm.damping_mask_factors()
+        full = sm.damping_mask
+        rebuilt = small[np.ix_(*index)]
+        assert rebuilt.dtype == full.dtype
+        assert np.array_equal(rebuilt, full)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_extension_index_rebuilds_the_extended_models(dim):
+    rng = np.random.default_rng(dim)
+    shape = (11, 14, 9)[:dim]
+    h = 5.0
+    box = []
+    for n in shape:
+        box += [0.0, (n - 1) * h]
+    vel = rng.uniform(1500, 3000, size=shape).astype(np.float32)
+    den = rng.uniform(1000, 2000, size=shape).astype(np.float32)
+    sm = sw.SpaceModel(bounding_box=tuple(box), grid_spacing=(h,) * dim,
+                       velocity_model=vel, density_model=den, space_order=6,
+                       dtype=np.float32)
+    sm.config_boundary(damping_length=tuple(float(w) * h for w in (3, 0, 5, 2, 1, 4)[:2 * dim]),
+                       boundary_condition=("null_dirichlet",) * (2 * dim))
+    index = sm.extension_index()
+    assert np.array_equal(sm.velocity_model[np.ix_(*index)], sm.extended_velocity_model)
+    assert np.array_equal(sm.density_model[np.ix_(*index)], sm.extended_density_model)

@@ -53,7 +53,7 @@ class FakeShot:
         self.left = max(0, self.left - n)
         return self.left
 
-    def results(self):
+    def results(self, stream=None):
         return ("u", self.req.ident), ("rec", self.req.ident)
 
     def release_plan(self):
