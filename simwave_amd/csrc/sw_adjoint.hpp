@@ -249,6 +249,89 @@ k_adj_update(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
     }
 }
 
+#if SW_DIM == 3
+// The same update (k_adj_update, same operations in the same order, so the
+// same results) marching z: each thread owns a (j, k) column of a z chunk
+// and keeps W's z neighbours in a register queue, so every W value is read
+// from memory once for its z terms instead of 2R+1 times.
+constexpr int kAdjZChunk = 16;
+
+template <int R>
+__global__ void __launch_bounds__(64 * kAdjLines)
+k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
+               const real_t *__restrict__ W, real_t *__restrict__ ub1,
+               real_t *__restrict__ ub2, const real_t *__restrict__ u0,
+               const real_t *__restrict__ u1, const real_t *__restrict__ u2,
+               double *__restrict__ grad, int image)
+{
+    const int j = blockIdx.y * kAdjLines + threadIdx.y;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (j >= P.nx || k >= P.ny)
+        return;
+    const int z0 = blockIdx.z * kAdjZChunk;
+    const int z1 = min(z0 + kAdjZChunk, P.nz);
+    const long long col = (long long)j * P.sx + k;
+    // wq[R + d] = W at plane i + d (zero outside the grid)
+    real_t wq[2 * R + 1];
+#pragma unroll
+    for (int d = -R; d < R; d++) {
+        const int z = z0 + d;
+        wq[R + d + 1] = (z >= 0 && z < P.nz) ? W[(long long)z * P.sz + col] : (real_t)0;
+    }
+    const bool in_xy = j >= R && j < P.nx - R && k >= R && k < P.ny - R;
+    for (int i = z0; i < z1; i++) {
+#pragma unroll
+        for (int d = 0; d < 2 * R; d++)
+            wq[d] = wq[d + 1];
+        wq[2 * R] = (i + R < P.nz) ? W[(long long)(i + R) * P.sz + col] : (real_t)0;
+        const long long p = (long long)i * P.sz + col;
+        // stencil_clipped's operations, z terms from the queue
+        const real_t c0 = wq[R] * P.c2[0];
+        real_t sz = c0, sx = c0, sy = c0;
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            const real_t cm = P.c2[m];
+            real_t t = 0;
+            if (i + m < P.nz) t += wq[R + m];
+            if (i - m >= 0) t += wq[R - m];
+            sz += cm * t;
+            t = 0;
+            if (j + m < P.nx) t += W[p + m * P.sx];
+            if (j - m >= 0) t += W[p - m * P.sx];
+            sx += cm * t;
+            t = 0;
+            if (k + m < P.ny) t += W[p + m];
+            if (k - m >= 0) t += W[p - m];
+            sy += cm * t;
+        }
+        const real_t lt = sy / P.dy2 + sx / P.dx2 + sz / P.dz2;
+        if (!(in_xy && i >= R && i < P.nz - R)) {
+            ub1[p] += lt;
+            ub[p] = 0;
+            continue;
+        }
+        const real_t w = ub[p];
+        ub[p] = 0;
+        const double e = (double)P.e[p];
+        const double rden = 1.0 / (1.0 + e);
+        const double a = 2.0 * rden, b = (1.0 - e) * rden;
+        ub1[p] = (real_t)((double)ub1[p] + a * (double)w + (double)lt);
+        ub2[p] = (real_t)((double)ub2[p] - b * (double)w);
+        if (w != (real_t)0) {
+            const double x0 = (double)u0[p], x1 = (double)u1[p], x2 = (double)u2[p];
+            if (image) {
+                grad[p] += (double)w * x0;
+            } else {
+                const double rv = 1.0 / (double)vel[p];
+                const double da = -4.0 * e * rden * rden * rv;
+                const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
+                grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+            }
+        }
+    }
+}
+#endif
+
 // Wavelet gradient of step n (k = n+1): gw[s] = sum_p kws_{s,p} c_p ub_p
 // (c = q / den) over the source box's interior points, in float64, one
 // block per source.  (The velocity derivative of the source term is part

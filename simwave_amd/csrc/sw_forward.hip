@@ -1486,6 +1486,43 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
 // `grad` (float64, one value per extended grid point) the velocity
 // gradient and writes into `gw` ([T][num_sources], device, or null) the
 // wavelet gradient.
+// Reverse interior update of one step: the z-marching kernel in 3D, the
+// whole-grid one in 2D (same operations, same results).
+#if SW_DIM == 3
+template <int R>
+void adj_update_z(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *ub,
+                  const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
+                  const real_t *u1, const real_t *u2, double *grad, int image)
+{
+    const dim3 grid((unsigned)((P.ny + 63) / 64),
+                    (unsigned)((P.nx + sw::kAdjLines - 1) / sw::kAdjLines),
+                    (unsigned)((P.nz + sw::kAdjZChunk - 1) / sw::kAdjZChunk));
+    hipLaunchKernelGGL(sw::k_adj_update_z<R>, grid, sw::adj_block(), 0, st, P, vel, ub, W,
+                       ub1, ub2, u0, u1, u2, grad, image);
+}
+#endif
+
+void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *ub,
+                const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
+                const real_t *u1, const real_t *u2, double *grad, int image)
+{
+#if SW_DIM == 3
+    switch (P.r) {
+#define SW_ADJ_CASE(R)                                                              \
+    case R:                                                                         \
+        adj_update_z<R>(P, st, vel, ub, W, ub1, ub2, u0, u1, u2, grad, image);      \
+        return;
+    SW_ADJ_CASE(1) SW_ADJ_CASE(2) SW_ADJ_CASE(3) SW_ADJ_CASE(4) SW_ADJ_CASE(5)
+    SW_ADJ_CASE(6) SW_ADJ_CASE(7) SW_ADJ_CASE(8) SW_ADJ_CASE(9) SW_ADJ_CASE(10)
+#undef SW_ADJ_CASE
+    default:
+        break;
+    }
+#endif
+    hipLaunchKernelGGL(sw::k_adj_update, sw::adj_grid(P), sw::adj_block(), 0, st, P, vel, ub, W,
+                       ub1, ub2, u0, u1, u2, grad, image);
+}
+
 double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *gw, size_t K,
                    hipStream_t st, unsigned flags)
 {
@@ -1611,10 +1648,8 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
             if (a.nsrc && gw)
                 hipLaunchKernelGGL(sw::k_adj_sources, dim3((unsigned)a.nsrc), dim3(256), 0, st,
                                    P, ub, gw + (n - 1) * a.nsrc);
-            hipLaunchKernelGGL(sw::k_adj_update, sw::adj_grid(P), sw::adj_block(), 0, st, P,
-                               a.vel, ub, W, ub1, ub2, seg + (n + 1 - m) * npts,
-                               seg + (n - m) * npts, seg + (n - 1 - m) * npts, grad,
-                               image ? 1 : 0);
+            adj_update(P, st, a.vel, ub, W, ub1, ub2, seg + (n + 1 - m) * npts,
+                       seg + (n - m) * npts, seg + (n - 1 - m) * npts, grad, image ? 1 : 0);
             hip_check(hipGetLastError(), "adjoint step");
             if (n == n_lo)
                 break;

@@ -264,6 +264,7 @@ ADJOINT_CASES = {
     "3d_mixed": dict(dim=3, n=18, space_order=4, tf=0.04, damping=30.0,
                      bc=("null_neumann", "null_dirichlet", "null_dirichlet",
                          "null_neumann", "none", "null_neumann")),
+    "3d_so8": dict(dim=3, shape=(22, 16, 19), space_order=8, tf=0.04, damping=20.0),
 }
 
 
