@@ -503,7 +503,7 @@ bool make_fixed_div(real_t b, sw::FixedDiv &d)
 // thread RPT, wavefronts per block NW).  Radius 4 (space order 8) carries
 // every configuration; other radii carry 0, 4 and 8.  SIMWAVE_HIP_TILE picks
 // one; otherwise default_tile(R).
-constexpr int kTileShapes = 14;
+constexpr int kTileShapes = 17;
 constexpr int default_tile(int R) { return R == 4 ? 10 : R < 4 ? 0 : 4; }
 
 template <int R, int WR, int RPT, int NW>
@@ -574,6 +574,9 @@ TiledKernel tiled_entry(int v)
         case 11: return tiled_config<R, 4, 1, 1>();
         case 12: return tiled_config<R, 4, 1, 8>();
         case 13: return tiled_config<R, 2, 1, 8>();
+        case 14: return tiled_config<R, 8, 2, 1>();
+        case 15: return tiled_config<R, 4, 2, 2>();
+        case 16: return tiled_config<R, 8, 2, 2>();
         default: return tiled_config<R, 1, 1, 4>();
         }
     } else {

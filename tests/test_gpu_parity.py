@@ -129,7 +129,7 @@ def test_parity_generic_path(gpu, name):
 
 
 # Tile configurations of the 3D tiled kernel (SIMWAVE_HIP_TILE): radius 4
-# carries 0..13, other radii 0, 4 and 8; SIMWAVE_HIP_ORDER picks the tile
+# carries 0..16, other radii 0, 4 and 8; SIMWAVE_HIP_ORDER picks the tile
 # order.
 TILED_R4 = ["3d_cd_so8", "3d_wide_so8", "3d_wide_f64_so8", "3d_wide_bc"]
 TILED_OTHER = ["3d_cd_so2", "3d_cd_so20", "3d_bc_mixed", "3d_src_halo", "3d_rect",
@@ -151,7 +151,7 @@ DEEP_R4 = ["3d_deep_so8", "3d_deep_so8_whole", "3d_deep_none", "3d_deep_f64"]
 
 
 @pytest.mark.parametrize("order", [0, 1])
-@pytest.mark.parametrize("tile", list(range(14)))
+@pytest.mark.parametrize("tile", list(range(17)))
 @pytest.mark.parametrize("name", DEEP_R4)
 def test_parity_clean_tiles_r4(gpu, name, tile, order):
     """Tiles away from every face and source (8-plane z chunks), every
@@ -174,7 +174,7 @@ def test_parity_clean_tiles_r2(gpu, tile, order):
 
 
 @pytest.mark.parametrize("order", [0, 1])
-@pytest.mark.parametrize("tile", list(range(14)))
+@pytest.mark.parametrize("tile", list(range(17)))
 @pytest.mark.parametrize("name", TILED_R4)
 def test_parity_tiled_configs_r4(gpu, name, tile, order):
     """Every tile configuration of the radius-4 tiled 3D kernel."""
