@@ -99,7 +99,11 @@ def points_and_values(grid_shape, positions, half_width):
     per_axis = []
     for axis in range(dim):
         n = int(grid_shape[axis])
-        pos = positions[:, axis]
+        # every quantity below is a function of the coordinate alone:
+        # evaluate it once per distinct coordinate (receiver lines and grids
+        # repeat them) and spread the rows back
+        pos, back = np.unique(positions[:, axis], return_inverse=True)
+        back = back.reshape(-1)
         lo = np.floor(pos.astype(np.float64)).astype(np.int64) - half_width - 1
         lo = np.clip(lo, 0, max(n - 1, 0))
         idx = lo[:, None] + np.arange(span)[None, :]
@@ -117,6 +121,7 @@ def points_and_values(grid_shape, positions, half_width):
                 "There is no valid point in the source/receiver location")
         first = valid.argmax(axis=1)
         last = span - 1 - valid[:, ::-1].argmax(axis=1)
+        lo, w, first, last = lo[back], w[back], first[back], last[back]
         starts[:, axis] = lo
         begins[:, axis] = lo + first
         ends[:, axis] = lo + last

@@ -524,3 +524,20 @@ def test_extension_index_rebuilds_the_extended_models(dim):
     index = sm.extension_index()
     assert np.array_equal(sm.velocity_model[np.ix_(*index)], sm.extended_velocity_model)
     assert np.array_equal(sm.density_model[np.ix_(*index)], sm.extended_density_model)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_points_and_values_with_repeated_coordinates(dtype):
+    """Receiver grids repeat coordinates per axis (evaluated once per
+    distinct value): same intervals and weights as per-position calls."""
+    rng = np.random.default_rng(11)
+    shape = (40, 57, 66)
+    xs = rng.uniform(5, 50, 7)
+    ys = rng.uniform(5, 60, 6)
+    pos = np.array([(3.25, x, y) for x in xs for y in ys] + [(3.25, xs[0], 9.0)] * 3,
+                   dtype=dtype)
+    pos = pos[rng.permutation(len(pos))]
+    pts, vals = kws.points_and_values(shape, pos, 4)
+    one = [kws.points_and_values(shape, pos[i:i + 1], 4) for i in range(len(pos))]
+    assert np.array_equal(pts, np.concatenate([p for p, _ in one]))
+    assert np.array_equal(vals, np.concatenate([v for _, v in one]))
