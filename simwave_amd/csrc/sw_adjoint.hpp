@@ -254,7 +254,7 @@ k_adj_update(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
 // same results) marching z: each thread owns a (j, k) column of a z chunk
 // and keeps W's z neighbours in a register queue, so every W value is read
 // from memory once for its z terms instead of 2R+1 times.
-constexpr int kAdjZChunk = 16;
+constexpr int kAdjZChunk = 8;
 
 template <int R>
 __global__ void __launch_bounds__(64 * kAdjLines)
@@ -279,13 +279,43 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
         wq[R + d + 1] = (z >= 0 && z < P.nz) ? W[(long long)z * P.sz + col] : (real_t)0;
     }
     const bool in_xy = j >= R && j < P.nx - R && k >= R && k < P.ny - R;
+    // Operands of one point: loaded a plane ahead (all in bounds), so the
+    // loads of plane i+1 overlap the arithmetic of plane i.
+    struct Ops {
+        real_t head, w, e, v, b1, b2, x0, x1, x2;
+        real_t xn[2 * R], yn[2 * R];
+    };
+    auto load = [&](int i, Ops &o) {
+        const long long p = (long long)i * P.sz + col;
+        o.head = (i + R < P.nz) ? W[p + R * P.sz] : (real_t)0;
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            o.xn[2 * m - 2] = (j + m < P.nx) ? W[p + m * P.sx] : (real_t)0;
+            o.xn[2 * m - 1] = (j - m >= 0) ? W[p - m * P.sx] : (real_t)0;
+            o.yn[2 * m - 2] = (k + m < P.ny) ? W[p + m] : (real_t)0;
+            o.yn[2 * m - 1] = (k - m >= 0) ? W[p - m] : (real_t)0;
+        }
+        o.w = ub[p];
+        o.e = P.e[p];
+        o.v = vel[p];
+        o.b1 = ub1[p];
+        o.b2 = ub2[p];
+        o.x0 = u0[p];
+        o.x1 = u1[p];
+        o.x2 = u2[p];
+    };
+    Ops cur, nxt;
+    load(z0, cur);
     for (int i = z0; i < z1; i++) {
+        if (i + 1 < z1)
+            load(i + 1, nxt);
 #pragma unroll
         for (int d = 0; d < 2 * R; d++)
             wq[d] = wq[d + 1];
-        wq[2 * R] = (i + R < P.nz) ? W[(long long)(i + R) * P.sz + col] : (real_t)0;
+        wq[2 * R] = cur.head;
         const long long p = (long long)i * P.sz + col;
-        // stencil_clipped's operations, z terms from the queue
+        // stencil_clipped's operations in its order, with the z terms from
+        // the queue and the x/y terms preloaded
         const real_t c0 = wq[R] * P.c2[0];
         real_t sz = c0, sx = c0, sy = c0;
 #pragma unroll
@@ -296,38 +326,38 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
             if (i - m >= 0) t += wq[R - m];
             sz += cm * t;
             t = 0;
-            if (j + m < P.nx) t += W[p + m * P.sx];
-            if (j - m >= 0) t += W[p - m * P.sx];
+            if (j + m < P.nx) t += cur.xn[2 * m - 2];
+            if (j - m >= 0) t += cur.xn[2 * m - 1];
             sx += cm * t;
             t = 0;
-            if (k + m < P.ny) t += W[p + m];
-            if (k - m >= 0) t += W[p - m];
+            if (k + m < P.ny) t += cur.yn[2 * m - 2];
+            if (k - m >= 0) t += cur.yn[2 * m - 1];
             sy += cm * t;
         }
         const real_t lt = sy / P.dy2 + sx / P.dx2 + sz / P.dz2;
-        if (!(in_xy && i >= R && i < P.nz - R)) {
-            ub1[p] += lt;
-            ub[p] = 0;
-            continue;
-        }
-        const real_t w = ub[p];
+        const real_t w = cur.w;
         ub[p] = 0;
-        const double e = (double)P.e[p];
-        const double rden = 1.0 / (1.0 + e);
-        const double a = 2.0 * rden, b = (1.0 - e) * rden;
-        ub1[p] = (real_t)((double)ub1[p] + a * (double)w + (double)lt);
-        ub2[p] = (real_t)((double)ub2[p] - b * (double)w);
-        if (w != (real_t)0) {
-            const double x0 = (double)u0[p], x1 = (double)u1[p], x2 = (double)u2[p];
-            if (image) {
-                grad[p] += (double)w * x0;
-            } else {
-                const double rv = 1.0 / (double)vel[p];
-                const double da = -4.0 * e * rden * rden * rv;
-                const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
-                grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+        if (!(in_xy && i >= R && i < P.nz - R)) {
+            ub1[p] = cur.b1 + lt;
+        } else {
+            const double e = (double)cur.e;
+            const double rden = 1.0 / (1.0 + e);
+            const double a = 2.0 * rden, b = (1.0 - e) * rden;
+            ub1[p] = (real_t)((double)cur.b1 + a * (double)w + (double)lt);
+            ub2[p] = (real_t)((double)cur.b2 - b * (double)w);
+            if (w != (real_t)0) {
+                const double x0 = (double)cur.x0, x1 = (double)cur.x1, x2 = (double)cur.x2;
+                if (image) {
+                    grad[p] += (double)w * x0;
+                } else {
+                    const double rv = 1.0 / (double)cur.v;
+                    const double da = -4.0 * e * rden * rden * rv;
+                    const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
+                    grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+                }
             }
         }
+        cur = nxt;
     }
 }
 #endif
