@@ -87,6 +87,9 @@ double simwave_hip_forward_device_2d_cd_f32(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_2d_cd_f32; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_2d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -192,6 +195,9 @@ double simwave_hip_forward_device_2d_cd_f64(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_2d_cd_f64; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_2d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -297,6 +303,9 @@ double simwave_hip_forward_device_2d_vd_f32(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_2d_vd_f32; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_2d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -402,6 +411,9 @@ double simwave_hip_forward_device_2d_vd_f64(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_2d_vd_f64; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_2d_vd_f64(
     double *u, double *velocity, double *density, double *damp,
@@ -507,6 +519,9 @@ double simwave_hip_forward_device_3d_cd_f32(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_3d_cd_f32; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_3d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -612,6 +627,9 @@ double simwave_hip_forward_device_3d_cd_f64(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_3d_cd_f64; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_3d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -717,6 +735,9 @@ double simwave_hip_forward_device_3d_vd_f32(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_3d_vd_f32; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_3d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -822,6 +843,9 @@ double simwave_hip_forward_device_3d_vd_f64(
  * chunks / destroy, for callers that interleave shots (the serving loop) or
  * re-run step ranges (checkpointed recomputation).  Arguments as
  * simwave_hip_forward_device_3d_vd_f64; its host arrays are read only here.
+ * flags as the device entry, plus 8: for stride 0, capture 48 steps once in a
+ * hipGraph and replay it for every aligned 48-step stretch of a run (the
+ * step number then comes from a device counter; results are identical).
  * Returns NULL on failure. */
 void *simwave_hip_plan_create_3d_vd_f64(
     double *u, double *velocity, double *density, double *damp,

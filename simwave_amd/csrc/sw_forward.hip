@@ -96,6 +96,10 @@ constexpr unsigned kFlagGeneric = 2u;      // force the generic kernels
 // many threads keeps each shot on its one stream instead, since event waits
 // between streams that share a hardware queue can then order into a cycle.
 constexpr unsigned kFlagOverlapReceivers = 4u;
+// persistent plans of stride-0 shots: capture kGraphSteps steps once in a
+// hipGraph and replay it (the step number comes from a device counter)
+constexpr unsigned kFlagGraph = 8u;
+constexpr size_t kGraphSteps = 48;   // a multiple of the 3-slot cycle
 // adjoint entry: accumulate the zero-lag cross-correlation image of the
 // forward and adjoint wavefields instead of the velocity gradient
 constexpr unsigned kAdjImage = 1u;
@@ -816,7 +820,8 @@ template <typename SlotPtr, typename Before, typename After, typename Swap,
 void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &plan,
                size_t i0, size_t i1, real_t *rec_dev, size_t rec_row0, unsigned flags,
                hipStream_t st, SlotPtr slot_ptr, Before before_step, After after_step,
-               Swap swap, Uploads uploads, RecStream *rec_stream = nullptr)
+               Swap swap, Uploads uploads, RecStream *rec_stream = nullptr,
+               size_t nshift = 0)
 {
     const Params &P = pr.P;
     const int r = pr.r;
@@ -871,7 +876,9 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
                                pr.rbox, pr.rw, (int)a.nrec, row);
     };
     for (size_t idx = i0; idx < i1; idx++) {
-        const size_t n = a.n0 + idx;
+        // nshift: capture of a graph replayed with a device step base
+        // (Params::nbase); kernels then see the step relative to it
+        const size_t n = a.n0 + idx - nshift;
         const StepSlots &s = plan[idx];
         const long long ii = (long long)(idx - i0);
         // the receivers of steps <= idx-2 are done before step idx; those of
@@ -1403,16 +1410,92 @@ struct Plan {
     size_t done = 0;             // plan entries enqueued so far
     hipEvent_t last = nullptr;   // end of the latest enqueued work
     RecStream rec;
+    // graph replay (kFlagGraph): kGraphSteps steps captured with the step
+    // number relative to *nbase; prg is pr with P.nbase set
+    Prepared prg;
+    int *nbase = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     Plan(const Args &args, hipStream_t s, unsigned f) : a(args), flags(f), st(s), arena(s) {}
     ~Plan()
     {
         // the arena frees on `st`: order it after the latest run
         if (last) {
+            if (gexec)
+                (void)hipEventSynchronize(last);   // replays done before destroying
             (void)hipStreamWaitEvent(st, last, 0);
             (void)hipEventDestroy(last);
         }
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
     }
 };
+
+// Whether the plan's slot cycle repeats every 3 steps with no exchanges
+// (stride 0), so that one captured chunk serves every aligned position.
+bool periodic_slots(const std::vector<StepSlots> &slots)
+{
+    for (size_t i = 0; i < slots.size(); i++) {
+        const StepSlots &s = slots[i], &b = slots[i % 3];
+        if (s.swap_after || s.prev != b.prev || s.cur != b.cur || s.next != b.next)
+            return false;
+    }
+    return true;
+}
+
+// Captures plan entries 0 .. kGraphSteps-1 on the plan's stream with the
+// step numbers relative to the device counter p.nbase (1 .. kGraphSteps);
+// a replay starting at entry e sets *nbase = n0 + e - 1 first.  The chunk's
+// kernels take the same arguments as in a direct run, except the
+// step-dependent ones (sources, receiver rows), which add *nbase.
+void capture_graph(Plan &p)
+{
+    p.nbase = p.arena.alloc<int>(1);
+    p.prg = p.pr;
+    p.prg.P.nbase = p.nbase;
+#if SW_DIM == 3
+    if (p.pr.tiled) {
+        Params *Pg = p.arena.alloc<Params>(1);
+        hip_check(hipMemcpyAsync(Pg, &p.prg.P, sizeof(Params), hipMemcpyHostToDevice, p.st),
+                  "copy params");
+        p.prg.Pd = Pg;
+    }
+#endif
+    // replays run after the plan's preparation (ordered by the plan's
+    // event); the capture records without executing, on a stream of its
+    // own (the default stream cannot be captured), at the default priority
+    struct OwnStream {
+        hipStream_t s = nullptr;
+        ~OwnStream()
+        {
+            if (s)
+                (void)hipStreamDestroy(s);
+        }
+    } cs;
+    hip_check(hipStreamCreateWithFlags(&cs.s, hipStreamNonBlocking), "capture stream");
+    hipStream_t st = cs.s;
+    const size_t npts = (size_t)p.pr.npts;
+    real_t *u = p.a.u;
+    auto slot_ptr = [&](size_t sl) { return u + sl * npts; };
+    auto nop = [](size_t) {};
+    auto noswap = [](size_t, size_t) {};
+    auto no_uploads = [](size_t) { return false; };
+    hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+        time_loop(p.a, p.prg, p.slots, 0, kGraphSteps, p.a.receivers, 0, 0u, st, slot_ptr,
+                  nop, nop, noswap, no_uploads, nullptr, p.a.n0 - 1);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g)
+            (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(st, &p.graph), "end capture");
+    hip_check(hipGraphInstantiate(&p.gexec, p.graph, nullptr, nullptr, 0), "graph instantiate");
+}
 
 Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
 {
@@ -1430,6 +1513,9 @@ Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
                  a.n0 + idx, a.num_snapshots);
     }
     p->pr = prepare(a, true, flags, p->arena, st);
+    if ((flags & kFlagGraph) && !(flags & (kFlagOverlapReceivers | kFlagTimeSteps)) &&
+        p->slots.size() >= kGraphSteps && periodic_slots(p->slots))
+        capture_graph(*p);
     hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventRecord(p->last, st), "hipEventRecord");
     return p.release();
@@ -1451,8 +1537,28 @@ size_t run_plan(Plan &p, size_t count, hipStream_t st)
                            u + y * npts, (long long)npts);
     };
     auto no_uploads = [](size_t) { return false; };
-    time_loop(p.a, p.pr, p.slots, i0, i1, p.a.receivers, 0, p.flags, st, slot_ptr, nop,
-              nop, swap, no_uploads, &p.rec);
+    size_t i = i0;
+    while (i < i1) {
+        if (p.gexec && i % 3 == 0 && i + kGraphSteps <= i1) {
+            // replay: entries i .. i+kGraphSteps-1 have the captured slots
+            hipLaunchKernelGGL(sw::k_set_int, dim3(1), dim3(1), 0, st, p.nbase,
+                               (int)(p.a.n0 + i - 1));
+            hip_check(hipGetLastError(), "step base");
+            hip_check(hipGraphLaunch(p.gexec, st), "graph launch");
+            i += kGraphSteps;
+            continue;
+        }
+        // direct launches up to the next replay position (or the end)
+        size_t j = i1;
+        if (p.gexec) {
+            const size_t next = (i + 2) / 3 * 3;
+            if (next > i && next + kGraphSteps <= i1)
+                j = next;
+        }
+        time_loop(p.a, p.pr, p.slots, i, j, p.a.receivers, 0, p.flags, st, slot_ptr, nop,
+                  nop, swap, no_uploads, &p.rec);
+        i = j;
+    }
     hip_check(hipEventRecord(p.last, st), "hipEventRecord");
     p.done = i1;
     g_timings[6] = p.pr.fused ? 1.0 : 0.0;

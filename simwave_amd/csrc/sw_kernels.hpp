@@ -64,6 +64,9 @@ struct Params {
     const Box *sbox;
     const real_t *sw;        // source weights
     const real_t *wavelet;
+    // graph replay: when set, the step number a kernel receives is relative
+    // and *nbase is added to it on the device (sources, receiver rows)
+    const int *nbase;
 };
 
 // ---------------------------------------------------------------- helpers
@@ -129,6 +132,8 @@ __device__ __forceinline__ real_t source_term(real_t q, real_t e, real_t kws,
 
 __device__ __forceinline__ real_t wavelet_amp(const Params &P, int n, int s)
 {
+    if (P.nbase)
+        n += *P.nbase;
     const long long idx = P.wav_multi ? (long long)(n - 1) * P.nsrc + s
                                       : (long long)(n - 1);
     return P.wavelet[idx];
@@ -739,6 +744,8 @@ __global__ void __launch_bounds__(256)
 k_receivers(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbox,
             const real_t *__restrict__ rw, int nrec, real_t *__restrict__ out)
 {
+    if (P.nbase)
+        out += (long long)(*P.nbase) * nrec;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nrec)
         return;
@@ -778,6 +785,8 @@ __global__ void __launch_bounds__(256)
 k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbox,
                  const real_t *__restrict__ rw, int nrec, real_t *__restrict__ out)
 {
+    if (P.nbase)
+        out += (long long)(*P.nbase) * nrec;
     __shared__ real_t prod[CH][RPB];
     __shared__ Box boxes[RPB];
     __shared__ int counts[RPB];
@@ -863,6 +872,14 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
 
 // Zero-fills a word array (a kernel rather than a memset, so it behaves the
 // same inside captured graphs).
+// The step base of a graph replay (a kernel, so that no memset node is
+// needed around graphs).
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *p = v;
+}
+
 __global__ void k_zero_u32(unsigned *p, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;

@@ -15,11 +15,13 @@ import numpy as np
 
 from .middleware import Middleware
 
-__all__ = ["DeviceForward", "FLAG_TIME_STEPS", "FLAG_GENERIC", "FLAG_OVERLAP_RECEIVERS"]
+__all__ = ["DeviceForward", "FLAG_TIME_STEPS", "FLAG_GENERIC", "FLAG_OVERLAP_RECEIVERS",
+           "FLAG_GRAPH"]
 
 FLAG_TIME_STEPS = 1   # accumulate the step kernel's time with HIP events
 FLAG_GENERIC = 2      # use the generic kernels instead of the fused one
 FLAG_OVERLAP_RECEIVERS = 4   # receiver gather of large grids on a second stream
+FLAG_GRAPH = 8               # plans of stride-0 shots: replay a captured 48-step hipGraph
 
 _TORCH_DTYPE = {"float32": None, "float64": None}
 

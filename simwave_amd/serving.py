@@ -10,7 +10,8 @@ while others run.  The server
   thread through the library's persistent plan (``simwave_hip_plan_*``),
   in chunks of time steps, so the kernels of different shots overlap on the
   device and their host-side launch work runs in parallel (the ctypes calls
-  release the GIL);
+  release the GIL); a stride-0 shot's chunks replay a hipGraph its plan
+  captured once;
 * frees a finished shot's memory at once, which admits the next waiting
   request, and resolves the request's future with ``(u_full, shot_record)``
   exactly as ``Solver.forward()`` returns them;
@@ -109,7 +110,10 @@ class ShotServer:
     max_active : int
         Shots running at once (one worker thread and stream each).
     chunk_steps : int
-        Time steps a worker enqueues per plan call.
+        Time steps a worker enqueues per plan call (a multiple of 48 lets
+        every full chunk of a stride-0 shot replay the plan's graph).
+    plan_flags : int, optional
+        Flags of the plans (default: FLAG_GRAPH, captured-graph replay).
     device : str
     shot_factory : callable, optional
         ``solver -> shot`` with the ``DeviceForward`` interface (``steps``,
@@ -117,11 +121,15 @@ class ShotServer:
         ``release_plan``); replaceable for tests, as is ``stream_factory``.
     """
 
-    def __init__(self, hbm_budget=200e9, max_active=16, chunk_steps=64, device="cuda",
-                 shot_factory=None, stream_factory=None):
+    def __init__(self, hbm_budget=200e9, max_active=16, chunk_steps=48, device="cuda",
+                 shot_factory=None, stream_factory=None, plan_flags=None):
         self.hbm_budget = float(hbm_budget)
         self.max_active = int(max_active)
         self.chunk_steps = int(chunk_steps)
+        if plan_flags is None:
+            from .device import FLAG_GRAPH
+            plan_flags = FLAG_GRAPH
+        self.plan_flags = int(plan_flags)
         self.device = device
         self._shot_factory = shot_factory or self._device_shot
         self._stream_factory = stream_factory or self._new_stream
@@ -224,7 +232,7 @@ class ShotServer:
             t0 = time.perf_counter()
             with ctx:
                 shot = self._shot_factory(req.solver)
-            shot.plan(stream)
+            shot.plan(stream, flags=self.plan_flags)
             t1 = time.perf_counter()
             left = shot.steps
             while left > 0:

@@ -37,7 +37,7 @@ class FakeShot:
         self.left = req.steps
         self.planned = False
 
-    def plan(self, stream):
+    def plan(self, stream, flags=0):
         with self.mon.lock:
             self.mon.used += self.req.device_bytes
             self.mon.active += 1
