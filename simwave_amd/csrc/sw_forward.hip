@@ -1586,15 +1586,6 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
 }
 
 // ------------------------------------------------------------- adjoint
-// Shot record and its adjoint: runs the forward of a constant-density shot
-// (device arrays, as the device entry; stride 0, steps 1..T) saving the two
-// time levels (u^m, u^{m+1}) every `K` steps, then walks the checkpoint
-// segments backwards: recomputes each segment's states, then applies the
-// transpose of every step (sw_adjoint.hpp) to the adjoint of the shot
-// record `rec_adj` ([T][num_receivers], device).  Accumulates into
-// `grad` (float64, one value per extended grid point) the velocity
-// gradient and writes into `gw` ([T][num_sources], device, or null) the
-// wavelet gradient.
 // Reverse interior update of one step: the z-marching kernel in 3D, the
 // whole-grid one in 2D (same operations, same results).
 #if SW_DIM == 3
@@ -1632,6 +1623,15 @@ void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *
                        ub1, ub2, u0, u1, u2, grad, image);
 }
 
+// Shot record and its adjoint: runs the forward of a constant-density shot
+// (device arrays, as the device entry; stride 0, steps 1..T) saving the two
+// time levels (u^m, u^{m+1}) every `K` steps, then walks the checkpoint
+// segments backwards: recomputes each segment's states, then applies the
+// transpose of every step (sw_adjoint.hpp) to the adjoint of the shot
+// record `rec_adj` ([T][num_receivers], device).  Accumulates into
+// `grad` (float64, one value per extended grid point) the velocity
+// gradient and writes into `gw` ([T][num_sources], device, or null) the
+// wavelet gradient.
 double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *gw, size_t K,
                    hipStream_t st, unsigned flags)
 {
