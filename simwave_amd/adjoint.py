@@ -15,6 +15,7 @@ vector-Jacobian product with respect to that tensor.  PyTorch is used
 only for device memory and the autograd plumbing.
 """
 import ctypes
+import functools
 
 import numpy as np
 
@@ -121,7 +122,10 @@ def misfit_gradient(solver, observed, checkpoint_every=20, velocity=None, device
     return misfit, grad.cpu().numpy()
 
 
+@functools.lru_cache(maxsize=None)
 def _autograd():
+    """The autograd Function class (built once, on first use, so that
+    importing this module does not import torch)."""
     import torch
 
     class ShotRecord(torch.autograd.Function):
