@@ -120,7 +120,8 @@ int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
@@ -228,7 +229,8 @@ int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
@@ -336,7 +338,8 @@ int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
@@ -444,7 +447,8 @@ int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
@@ -552,7 +556,8 @@ int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
@@ -660,7 +665,8 @@ int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
@@ -768,7 +774,8 @@ int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
@@ -876,7 +883,8 @@ int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
  * float64 [extended grid] (device), set to the gradient with respect to
  * the velocity; grad_wavelet, [T][num_sources] (device) or NULL, set to
  * the gradient with respect to the wavelet samples; checkpoint_every, the
- * forward states saved every that many steps and recomputed per segment.
+ * forward states saved every that many steps and recomputed per segment
+ * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).

@@ -2,9 +2,10 @@
 
 ``gradient`` runs the library's adjoint entry
 (``simwave_hip_gradient_<variant>``, include/simwave_hip.h): the forward of
-a constant-density shot with checkpoints every ``checkpoint_every`` steps,
-then the transpose of every time step, backwards, with each checkpoint
-segment recomputed from its two saved time levels.  Given the adjoint of
+a constant-density shot, then the transpose of every time step, backwards.
+When every time level fits in HBM the forward keeps them all; otherwise it
+saves checkpoints every ``checkpoint_every`` steps and each segment is
+recomputed from its two saved levels.  Given the adjoint of
 the shot record (for the L2 misfit 1/2 ||d - d_obs||^2 that is the
 residual d - d_obs) it returns the gradient with respect to the extended
 velocity model and with respect to the wavelet samples.
@@ -22,7 +23,7 @@ import numpy as np
 from .device import DeviceForward
 from .middleware import _last_error
 
-__all__ = ["gradient", "misfit_gradient", "image", "ShotRecord"]
+__all__ = ["gradient", "misfit_gradient", "image", "ShotRecord", "checkpoint_interval"]
 
 IMAGE = 1     # adjoint entry flag: cross-correlation image instead of gradient
 OVERLAP_RECEIVERS = 4   # forward receiver gather on a second stream (large grids)
@@ -35,10 +36,25 @@ def _check(solver):
         raise ValueError("the adjoint needs saving_stride 0")
 
 
+def checkpoint_interval(dev, checkpoint_every=None):
+    """The interval the entry gets: `checkpoint_every` if given; otherwise
+    every time level kept in HBM (no recomputation) when they fit in 80% of
+    the free device memory, else checkpoints every 20 steps."""
+    if checkpoint_every is not None:
+        return int(checkpoint_every)
+    import torch
+    T = dev.kwargs["end_timestep"]
+    vel = dev.tensors["velocity_model"]
+    need = (T + 2) * vel.numel() * vel.element_size()
+    free = torch.cuda.mem_get_info(dev.device)[0]
+    return T if need <= 0.8 * free else 20
+
+
 def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=False,
          overlap_receivers=True):
     import torch
     kw = dev.kwargs
+    checkpoint_every = checkpoint_interval(dev, checkpoint_every)
     lib = dev._lib
     fn = getattr(lib, "simwave_hip_gradient_" + dev.variant)
     fn.restype = ctypes.c_double
@@ -66,7 +82,7 @@ def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=Fa
     return grad, gw
 
 
-def gradient(solver, rec_adjoint, checkpoint_every=20, wavelet_gradient=False,
+def gradient(solver, rec_adjoint, checkpoint_every=None, wavelet_gradient=False,
              velocity=None, device="cuda"):
     """Shot record and adjoint gradients of one shot.
 
@@ -76,6 +92,10 @@ def gradient(solver, rec_adjoint, checkpoint_every=20, wavelet_gradient=False,
         Constant density, saving_stride 0.
     rec_adjoint : array [timesteps, receivers]
         Adjoint of the shot record (the residual for the L2 misfit).
+    checkpoint_every : int, optional
+        Forward levels saved every that many steps and recomputed per
+        segment; >= timesteps keeps every level.  Default: every level when
+        they fit in HBM (see checkpoint_interval), else 20.
     velocity : torch tensor, optional
         Extended velocity on the device (default: the solver's model).
 
@@ -95,7 +115,7 @@ def gradient(solver, rec_adjoint, checkpoint_every=20, wavelet_gradient=False,
     return out
 
 
-def image(solver, rec_adjoint, checkpoint_every=20, velocity=None, device="cuda"):
+def image(solver, rec_adjoint, checkpoint_every=None, velocity=None, device="cuda"):
     """Zero-lag cross-correlation image sum_k u^k lambda^k of the forward
     wavefield u and the adjoint wavefield lambda driven by `rec_adjoint`
     (reverse-time migration of the data `rec_adjoint`), float64 over the
@@ -106,7 +126,7 @@ def image(solver, rec_adjoint, checkpoint_every=20, velocity=None, device="cuda"
     return img.cpu().numpy()
 
 
-def misfit_gradient(solver, observed, checkpoint_every=20, velocity=None, device="cuda"):
+def misfit_gradient(solver, observed, checkpoint_every=None, velocity=None, device="cuda"):
     """1/2 ||d - observed||^2 and its gradient with respect to the extended
     velocity (float64, extended shape)."""
     _check(solver)
@@ -134,7 +154,7 @@ def _autograd():
         `velocity_ext` (device, model dtype); differentiable in it."""
 
         @staticmethod
-        def forward(ctx, velocity_ext, solver, checkpoint_every=20):
+        def forward(ctx, velocity_ext, solver, checkpoint_every=None):
             _check(solver)
             vel = velocity_ext.detach().contiguous()
             dev = DeviceForward(solver, device=vel.device, velocity=vel)

@@ -1672,13 +1672,17 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     const Params &P = pr.P;
     const size_t npts = (size_t)pr.npts;
     const size_t T = a.n1;
-    const size_t ncp = (T + K - 1) / K;
+    // K >= T: every level stays in HBM (no checkpoints, no recomputation)
+    const bool store_all = K >= T;
+    if (store_all)
+        K = T;
+    const size_t ncp = store_all ? 1 : (T + K - 1) / K;
     const int nb = (int)std::min<long long>(((long long)npts + 255) / 256, 16384);
     auto zero = [&](real_t *p, size_t count) {
         hipLaunchKernelGGL(sw::k_fill, dim3(nb), dim3(256), 0, st, p, (long long)count,
                            (real_t)0);
     };
-    real_t *cp = arena.alloc<real_t>(2 * ncp * npts);
+    real_t *cp = store_all ? nullptr : arena.alloc<real_t>(2 * ncp * npts);
     real_t *seg = arena.alloc<real_t>((K + 2) * npts);
     real_t *ubar = arena.alloc<real_t>(3 * npts);
     real_t *W = arena.alloc<real_t>(npts);
@@ -1708,8 +1712,23 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     auto nop = [](size_t) {};
     auto noswap = [](size_t, size_t) {};
     auto no_uploads = [](size_t) { return false; };
-    time_loop(a, pr, plan, 0, plan.size(), a.receivers, 0, flags & kFlagOverlapReceivers, st,
-              slot_ptr, save, nop, noswap, no_uploads);
+    auto seg_ptr = [&](size_t l) { return seg + l * npts; };
+    if (store_all) {
+        // level l of the run in seg + l*npts
+        std::vector<StepSlots> sp;
+        for (size_t n = 1; n <= T; n++)
+            sp.push_back(StepSlots{n - 1, n, n + 1, false, 0, 0});
+        time_loop(a, pr, sp, 0, sp.size(), a.receivers, 0, flags & kFlagOverlapReceivers, st,
+                  seg_ptr, nop, nop, noswap, no_uploads);
+        // the caller's slots end as after a plain forward: level l in l % 3
+        for (size_t l = T - 1; l <= T + 1; l++)
+            hip_check(hipMemcpyAsync(slot_ptr(l % 3), seg_ptr(l), npts * sizeof(real_t),
+                                     hipMemcpyDeviceToDevice, st),
+                      "final levels");
+    } else {
+        time_loop(a, pr, plan, 0, plan.size(), a.receivers, 0, flags & kFlagOverlapReceivers,
+                  st, slot_ptr, save, nop, noswap, no_uploads);
+    }
     const double t_fwd = now_s();
 
     // backward over the checkpoint segments
@@ -1718,16 +1737,18 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     for (size_t j = ncp; j-- > 0;) {
         const size_t m = j * K;                   // levels m, m+1 saved
         const size_t n_lo = m + 1, n_hi = std::min(m + K, T);
-        hip_check(hipMemcpyAsync(seg, cp + (2 * j) * npts, 2 * npts * sizeof(real_t),
-                                 hipMemcpyDeviceToDevice, st), "restore checkpoint");
-        std::vector<StepSlots> sp;
-        for (size_t n = n_lo; n <= n_hi; n++)
-            sp.push_back(StepSlots{n - 1 - m, n - m, n + 1 - m, false, 0, 0});
-        seg_args.n0 = n_lo;
-        seg_args.n1 = n_hi;
-        auto seg_ptr = [&](size_t l) { return seg + l * npts; };
-        time_loop(seg_args, pr, sp, 0, sp.size(), nullptr, 0, 0u, st, seg_ptr, nop, nop,
-                  noswap, no_uploads);
+        if (!store_all) {
+            hip_check(hipMemcpyAsync(seg, cp + (2 * j) * npts, 2 * npts * sizeof(real_t),
+                                     hipMemcpyDeviceToDevice, st),
+                      "restore checkpoint");
+            std::vector<StepSlots> sp;
+            for (size_t n = n_lo; n <= n_hi; n++)
+                sp.push_back(StepSlots{n - 1 - m, n - m, n + 1 - m, false, 0, 0});
+            seg_args.n0 = n_lo;
+            seg_args.n1 = n_hi;
+            time_loop(seg_args, pr, sp, 0, sp.size(), nullptr, 0, 0u, st, seg_ptr, nop, nop,
+                      noswap, no_uploads);
+        }
         for (size_t n = n_hi; n >= n_lo; n--) {
             const size_t k = n + 1;
             real_t *ub = ubar + (k % 3) * npts;

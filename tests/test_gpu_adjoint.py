@@ -162,3 +162,18 @@ def test_library_rejects_bad_arguments():
         adjoint.gradient(solver, rr, checkpoint_every=0)
     with pytest.raises(ValueError, match="shape"):
         adjoint.gradient(solver, rr[:-1])
+
+
+def test_default_keeps_every_level_when_it_fits():
+    """Small shots fit in HBM: the default interval keeps every level (no
+    recomputation) and gives the checkpointed result."""
+    _, solver, kw = cases.adjoint_shot("3d_so8")
+    T = kw["end_timestep"]
+    dev = DeviceForward(solver)
+    assert adjoint.checkpoint_interval(dev) == T
+    rr = np.random.default_rng(10).standard_normal((T, kw["num_receivers"]))
+    a = adjoint.gradient(solver, rr, wavelet_gradient=True)
+    b = adjoint.gradient(solver, rr, checkpoint_every=5, wavelet_gradient=True)
+    np.testing.assert_array_equal(a["shot_record"], b["shot_record"])
+    assert harness.rel_l2(a["grad_velocity"], b["grad_velocity"]) < 1e-13
+    assert harness.rel_l2(a["grad_wavelet"], b["grad_wavelet"]) < 1e-13
