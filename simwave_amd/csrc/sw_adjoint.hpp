@@ -250,10 +250,15 @@ k_adj_update(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
 }
 
 #if SW_DIM == 3
-// The same update (k_adj_update, same operations in the same order, so the
-// same results) marching z: each thread owns a (j, k) column of a z chunk
-// and keeps W's z neighbours in a register queue, so every W value is read
-// from memory once for its z terms instead of 2R+1 times.
+// The update of k_adj_update marching z: each thread owns a (j, k) column
+// of a z chunk and keeps W's z neighbours in a register queue, so every W
+// value is read from memory once for its z terms instead of 2R+1 times.
+// The stencil and update take k_adj_update's operations in its order, with
+// one difference: ub2 (the adjoint of u^{n-1}) is written, not accumulated.
+// That buffer's last role was as ub of step n+1, and between its
+// consumption there and this write nothing adds to its interior; so ub
+// needs clearing only on the halo (its interior is overwritten as the next
+// step's ub2 before anything reads it).
 constexpr int kAdjZChunk = 8;
 
 template <int R>
@@ -282,7 +287,7 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
     // Operands of one point: loaded a plane ahead (all in bounds), so the
     // loads of plane i+1 overlap the arithmetic of plane i.
     struct Ops {
-        real_t head, w, e, v, b1, b2, x0, x1, x2;
+        real_t head, w, e, v, b1, x0, x1, x2;
         real_t xn[2 * R], yn[2 * R];
     };
     auto load = [&](int i, Ops &o) {
@@ -299,7 +304,6 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
         o.e = P.e[p];
         o.v = vel[p];
         o.b1 = ub1[p];
-        o.b2 = ub2[p];
         o.x0 = u0[p];
         o.x1 = u1[p];
         o.x2 = u2[p];
@@ -336,15 +340,15 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
         }
         const real_t lt = sy / P.dy2 + sx / P.dx2 + sz / P.dz2;
         const real_t w = cur.w;
-        ub[p] = 0;
         if (!(in_xy && i >= R && i < P.nz - R)) {
             ub1[p] = cur.b1 + lt;
+            ub[p] = 0;
         } else {
             const double e = (double)cur.e;
             const double rden = 1.0 / (1.0 + e);
             const double a = 2.0 * rden, b = (1.0 - e) * rden;
             ub1[p] = (real_t)((double)cur.b1 + a * (double)w + (double)lt);
-            ub2[p] = (real_t)((double)cur.b2 - b * (double)w);
+            ub2[p] = (real_t)(-b * (double)w);
             if (w != (real_t)0) {
                 const double x0 = (double)cur.x0, x1 = (double)cur.x1, x2 = (double)cur.x2;
                 if (image) {
