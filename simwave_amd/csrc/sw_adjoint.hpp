@@ -366,6 +366,106 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
 }
 #endif
 
+#if SW_DIM == 2
+// 2D form of k_adj_update_z: each thread owns an x column of a z chunk
+// (x is the contiguous axis), W's z neighbours in a register queue, the
+// next row's operands loaded while the current one is computed, and ub2
+// written rather than accumulated (see k_adj_update_z).
+constexpr int kAdjZChunk = 8;
+
+template <int R>
+__global__ void __launch_bounds__(256)
+k_adj_update_z2d(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
+                 const real_t *__restrict__ W, real_t *__restrict__ ub1,
+                 real_t *__restrict__ ub2, const real_t *__restrict__ u0,
+                 const real_t *__restrict__ u1, const real_t *__restrict__ u2,
+                 double *__restrict__ grad, int image)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.nx)
+        return;
+    const int z0 = blockIdx.y * kAdjZChunk;
+    const int z1 = min(z0 + kAdjZChunk, P.nz);
+    real_t wq[2 * R + 1];
+#pragma unroll
+    for (int d = -R; d < R; d++) {
+        const int z = z0 + d;
+        wq[R + d + 1] = (z >= 0 && z < P.nz) ? W[(long long)z * P.sz + j] : (real_t)0;
+    }
+    const bool in_x = j >= R && j < P.nx - R;
+    struct Ops {
+        real_t head, w, e, v, b1, x0, x1, x2;
+        real_t xn[2 * R];
+    };
+    auto load = [&](int i, Ops &o) {
+        const long long p = (long long)i * P.sz + j;
+        o.head = (i + R < P.nz) ? W[p + R * P.sz] : (real_t)0;
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            o.xn[2 * m - 2] = (j + m < P.nx) ? W[p + m] : (real_t)0;
+            o.xn[2 * m - 1] = (j - m >= 0) ? W[p - m] : (real_t)0;
+        }
+        o.w = ub[p];
+        o.e = P.e[p];
+        o.v = vel[p];
+        o.b1 = ub1[p];
+        o.x0 = u0[p];
+        o.x1 = u1[p];
+        o.x2 = u2[p];
+    };
+    Ops cur, nxt;
+    load(z0, cur);
+    for (int i = z0; i < z1; i++) {
+        if (i + 1 < z1)
+            load(i + 1, nxt);
+#pragma unroll
+        for (int d = 0; d < 2 * R; d++)
+            wq[d] = wq[d + 1];
+        wq[2 * R] = cur.head;
+        const long long p = (long long)i * P.sz + j;
+        // stencil_clipped's operations in its order (2D: x then z sums)
+        const real_t c0 = wq[R] * P.c2[0];
+        real_t sz = c0, sx = c0;
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            const real_t cm = P.c2[m];
+            real_t t = 0;
+            if (i + m < P.nz) t += wq[R + m];
+            if (i - m >= 0) t += wq[R - m];
+            sz += cm * t;
+            t = 0;
+            if (j + m < P.nx) t += cur.xn[2 * m - 2];
+            if (j - m >= 0) t += cur.xn[2 * m - 1];
+            sx += cm * t;
+        }
+        const real_t lt = sx / P.dx2 + sz / P.dz2;
+        const real_t w = cur.w;
+        if (!(in_x && i >= R && i < P.nz - R)) {
+            ub1[p] = cur.b1 + lt;
+            ub[p] = 0;
+        } else {
+            const double e = (double)cur.e;
+            const double rden = 1.0 / (1.0 + e);
+            const double a = 2.0 * rden, b = (1.0 - e) * rden;
+            ub1[p] = (real_t)((double)cur.b1 + a * (double)w + (double)lt);
+            ub2[p] = (real_t)(-b * (double)w);
+            if (w != (real_t)0) {
+                const double x0 = (double)cur.x0, x1 = (double)cur.x1, x2 = (double)cur.x2;
+                if (image) {
+                    grad[p] += (double)w * x0;
+                } else {
+                    const double rv = 1.0 / (double)cur.v;
+                    const double da = -4.0 * e * rden * rden * rv;
+                    const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
+                    grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+                }
+            }
+        }
+        cur = nxt;
+    }
+}
+#endif
+
 // Wavelet gradient of step n (k = n+1): gw[s] = sum_p kws_{s,p} c_p ub_p
 // (c = q / den) over the source box's interior points, in float64, one
 // block per source.  (The velocity derivative of the source term is part

@@ -1586,8 +1586,8 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
 }
 
 // ------------------------------------------------------------- adjoint
-// Reverse interior update of one step: the z-marching kernel in 3D, the
-// whole-grid one in 2D (same operations, same results).
+// Reverse interior update of one step: the z-marching kernels (3D, 2D);
+// the whole-grid k_adj_update for radii they do not carry.
 #if SW_DIM == 3
 template <int R>
 void adj_update_z(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *ub,
@@ -1611,6 +1611,21 @@ void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *
 #define SW_ADJ_CASE(R)                                                              \
     case R:                                                                         \
         adj_update_z<R>(P, st, vel, ub, W, ub1, ub2, u0, u1, u2, grad, image);      \
+        return;
+    SW_ADJ_CASE(1) SW_ADJ_CASE(2) SW_ADJ_CASE(3) SW_ADJ_CASE(4) SW_ADJ_CASE(5)
+    SW_ADJ_CASE(6) SW_ADJ_CASE(7) SW_ADJ_CASE(8) SW_ADJ_CASE(9) SW_ADJ_CASE(10)
+#undef SW_ADJ_CASE
+    default:
+        break;
+    }
+#else
+    switch (P.r) {
+#define SW_ADJ_CASE(R)                                                              \
+    case R:                                                                         \
+        hipLaunchKernelGGL(sw::k_adj_update_z2d<R>,                                \
+                           dim3((unsigned)((P.nx + 255) / 256),                     \
+                                (unsigned)((P.nz + sw::kAdjZChunk - 1) / sw::kAdjZChunk)), \
+                           dim3(256), 0, st, P, vel, ub, W, ub1, ub2, u0, u1, u2, grad, image); \
         return;
     SW_ADJ_CASE(1) SW_ADJ_CASE(2) SW_ADJ_CASE(3) SW_ADJ_CASE(4) SW_ADJ_CASE(5)
     SW_ADJ_CASE(6) SW_ADJ_CASE(7) SW_ADJ_CASE(8) SW_ADJ_CASE(9) SW_ADJ_CASE(10)
