@@ -3,6 +3,13 @@
 Float64 CPU restatement of the constant-density forward operator and its
 adjoint-state gradient, for small grids.  Only tests/ may import it.
 
+Pinned (tests/test_adjoint_oracle.py) by checks that do not restate it:
+its forward against the C oracle (oracle/sw_oracle.c, itself bit-identical
+to the reference C); its velocity gradient against central finite
+differences of its own misfit; its wavelet adjoint by the dot-product test
+<F s, r> = <s, F^T r>; its image as the derivative of <r, d> under a
+scaling of every new level.
+
 Forward (simwave/kernel/backend/c_code/forward/constant_density/{2d,3d}/
 wave.c, sections 1-4): per step n = 1..T, with q = dt^2/s, e = damp dt/(2s),
 s = 1/v^2, den = 1 + e:
