@@ -274,6 +274,8 @@ struct Prepared {
     int r = 0;
     long long npts = 0;
 #if SW_DIM == 3
+    bool zmarch = false;         // use k_step3d_z (SIMWAVE_HIP_ZMARCH=1)
+    int zmarch_zlen = 16;
     bool tiled = false;          // use k_step3d_vec
     void (*tiled_launch)(const struct Prepared &, const real_t *, const real_t *,
                          real_t *, int, hipStream_t) = nullptr;
@@ -458,8 +460,16 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
 #endif
     pr.fused = fused;
 #if SW_DIM == 3 && !SW_VD
-    if (fused && !env_flag("SIMWAVE_HIP_UNTILED"))
-        setup_tiling(pr, arena, st);
+    // the tiled kernel where its 32-bit buffer offsets fit; the z-marching
+    // one-point-per-thread kernel otherwise (or when asked for)
+    if (fused && !env_flag("SIMWAVE_HIP_UNTILED")) {
+        if (!env_flag("SIMWAVE_HIP_ZMARCH"))
+            setup_tiling(pr, arena, st);
+        if (!pr.tiled) {
+            pr.zmarch = true;
+            pr.zmarch_zlen = std::max(1, std::min(env_int("SIMWAVE_HIP_ZLEN", 8), 256));
+        }
+    }
 #endif
     return pr;
 }
@@ -800,6 +810,36 @@ struct RecStream {
     }
 };
 
+#if SW_DIM == 3
+// One step of k_step3d_z over the interior (constant density).
+void zmarch_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un, int n,
+                   hipStream_t st)
+{
+#if !SW_VD
+    const Params &P = pr.P;
+    const int R = pr.r, zl = pr.zmarch_zlen;
+    const dim3 grid((unsigned)((P.ny - 2 * R + 63) / 64),
+                    (unsigned)((P.nx - 2 * R + sw::kZLines - 1) / sw::kZLines),
+                    (unsigned)((P.nz - 2 * R + zl - 1) / zl));
+    const dim3 block(64, sw::kZLines);
+    switch (R) {
+#define SW_Z_CASE(RR)                                                                   \
+    case RR:                                                                            \
+        hipLaunchKernelGGL(sw::k_step3d_z<RR>, grid, block, 0, st, P, uc, up, un, n, zl); \
+        return;
+    SW_Z_CASE(1) SW_Z_CASE(2) SW_Z_CASE(3) SW_Z_CASE(4) SW_Z_CASE(5)
+    SW_Z_CASE(6) SW_Z_CASE(7) SW_Z_CASE(8) SW_Z_CASE(9) SW_Z_CASE(10)
+#undef SW_Z_CASE
+    default:
+        fail("stencil radius %d out of range", R);
+    }
+#else
+    (void)pr; (void)uc; (void)up; (void)un; (void)n; (void)st;
+    fail("the z-marching kernel covers constant density only");
+#endif
+}
+#endif
+
 // Runs steps [a.n0, a.n1].  `slot_ptr(i)` returns the device buffer backing
 // logical slot i; `before_step`/`after_step` manage residency (host entry;
 // `uploads(idx)` tells whether before_step(idx) writes a device buffer) and
@@ -905,7 +945,9 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
             if (timed)
                 hip_check(hipEventRecord(t0, st), "hipEventRecord");
 #if SW_DIM == 3
-            if (pr.tiled)
+            if (pr.zmarch)
+                zmarch_launch(pr, uc, up, un, (int)n, st);
+            else if (pr.tiled)
                 pr.tiled_launch(pr, uc, up, un, (int)n, st);
             else
 #endif

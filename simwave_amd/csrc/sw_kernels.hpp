@@ -437,6 +437,83 @@ k_step3d(Params P, const real_t *__restrict__ uc, const real_t *up,
 }
 #endif
 
+#if SW_DIM == 3 && !SW_VD
+// Fused 3D constant-density step, one point per thread: a thread owns a
+// (j, k) column of a z chunk, keeps u's z neighbours in a register queue and
+// loads every operand of plane i+1 (queue head, x/y neighbours, u_prev, q,
+// damping term) while plane i is computed; x/y neighbours come through the
+// caches.  Same operations in the same order as k_step3d.  Plain 64-bit
+// addressing: it serves grids whose planes are too large for the tiled
+// kernel's 32-bit buffer offsets (about 13% slower than the tiled kernel at
+// 472^3).
+constexpr int kZLines = 8;
+
+template <int R>
+__global__ void __launch_bounds__(64 * kZLines)
+k_step3d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ up,
+           real_t *__restrict__ un, int n, int zlen)
+{
+    const int k = R + blockIdx.x * 64 + threadIdx.x;
+    const int j = R + blockIdx.y * kZLines + threadIdx.y;
+    if (k >= P.ny - R || j >= P.nx - R)
+        return;
+    const int z0 = R + blockIdx.z * zlen;
+    const int z1 = min(z0 + zlen, P.nz - R);
+    const long long sz = P.sz, sx = P.sx;
+    const long long col = (long long)j * sx + k;
+    real_t qz[2 * R + 1];
+#pragma unroll
+    for (int m = 0; m < 2 * R; m++)
+        qz[m + 1] = uc[(long long)(z0 - R + m) * sz + col];
+    struct Ops {
+        real_t head, up, q, e;
+        real_t yn[2 * R], xn[2 * R];
+    };
+    auto load = [&](int i, Ops &o) {
+        const long long p = (long long)i * sz + col;
+        o.head = uc[p + R * sz];
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            o.yn[2 * m - 2] = uc[p + m];
+            o.yn[2 * m - 1] = uc[p - m];
+            o.xn[2 * m - 2] = uc[p + m * sx];
+            o.xn[2 * m - 1] = uc[p - m * sx];
+        }
+        o.up = up[p];
+        o.q = P.q[p];
+        o.e = P.e[p];
+    };
+    Ops cur, nxt;
+    load(z0, cur);
+    for (int i = z0; i < z1; i++) {
+        if (i + 1 < z1)
+            load(i + 1, nxt);
+#pragma unroll
+        for (int m = 0; m < 2 * R; m++)
+            qz[m] = qz[m + 1];
+        qz[2 * R] = cur.head;
+        const long long p = (long long)i * sz + col;
+        const real_t c = qz[R];
+        real_t acc_c = P.c2[0] * c;
+        real_t acc_m = P.c2[0] * c;
+        real_t acc_z = P.c2[0] * c;
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            acc_c += P.c2[m] * (cur.yn[2 * m - 2] + cur.yn[2 * m - 1]);
+            acc_m += P.c2[m] * (cur.xn[2 * m - 2] + cur.xn[2 * m - 1]);
+            acc_z += P.c2[m] * (qz[R + m] + qz[R - m]);
+        }
+        real_t value = 0;
+        value += acc_c / P.dy2 + acc_m / P.dx2 + acc_z / P.dz2;
+        real_t v = leapfrog(value, cur.q, cur.e, c, cur.up);
+        if (P.nsrc)
+            v = add_sources(P, n, i, j, k, p, v);
+        store_with_bc(P, un, p, i, j, k, v);
+        cur = nxt;
+    }
+}
+#endif
+
 // ------------------------------------------------ fused step, 2D
 // One workgroup = TB points along the contiguous axis x, marching along z.
 #if SW_DIM == 2

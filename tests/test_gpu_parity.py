@@ -234,3 +234,29 @@ def test_host_entry_keeps_initial_halo_values(gpu, name):
     (u_h, r_h), (u_o, r_o) = outs
     assert np.array_equal(u_h, u_o)
     assert np.array_equal(r_h, r_o)
+
+
+# The one-point-per-thread z-marching kernel (k_step3d_z): forced on for
+# representative cases, and taken by itself where planes are too large for
+# the tiled kernel's 32-bit buffer offsets.
+ZMARCH = ["3d_cd_so2", "3d_cd_so8", "3d_cd_so20", "3d_cd_f64_so8", "3d_bc_neumann",
+          "3d_bc_mixed", "3d_src_halo", "3d_wide_bc", "3d_deep_so8"]
+
+
+@pytest.mark.parametrize("name", ZMARCH)
+def test_parity_zmarch_kernel(gpu, name):
+    print("case", name, "(z-marching kernel)", flush=True)
+    os.environ["SIMWAVE_HIP_ZMARCH"] = "1"
+    try:
+        _compare(name, CASES[name], generic=False)
+    finally:
+        os.environ.pop("SIMWAVE_HIP_ZMARCH", None)
+
+
+def test_parity_planes_beyond_tiled_offsets(gpu):
+    """1920 x 1920 float64 planes (29.5 MB): the tiled kernel's offsets do
+    not fit, the z-marching kernel runs."""
+    params = dict(dim=3, shape=(14, 1912, 1912), space_order=8, dtype=F64, damping=0.0,
+                  tf=0.004)
+    print("case big planes", params["shape"], flush=True)
+    _compare("3d_big_planes", params, generic=False)
