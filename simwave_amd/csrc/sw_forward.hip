@@ -290,6 +290,7 @@ struct Prepared {
 
 #if SW_DIM == 3
 void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st);
+bool prefer_zmarch(const Prepared &pr);
 #endif
 
 bool env_flag(const char *name)
@@ -460,10 +461,14 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
 #endif
     pr.fused = fused;
 #if SW_DIM == 3 && !SW_VD
-    // the tiled kernel where its 32-bit buffer offsets fit; the z-marching
-    // one-point-per-thread kernel otherwise (or when asked for)
+    // the z-marching one-point-per-thread kernel where it is the faster one
+    // (prefer_zmarch), when asked for, or where the tiled kernel's 32-bit
+    // buffer offsets do not fit; the tiled kernel otherwise or when a tile
+    // configuration is asked for (SIMWAVE_HIP_TILE)
     if (fused && !env_flag("SIMWAVE_HIP_UNTILED")) {
-        if (!env_flag("SIMWAVE_HIP_ZMARCH"))
+        const char *tile = getenv("SIMWAVE_HIP_TILE");
+        const bool want_tile = tile && *tile;
+        if (!env_flag("SIMWAVE_HIP_ZMARCH") && (want_tile || !prefer_zmarch(pr)))
             setup_tiling(pr, arena, st);
         if (!pr.tiled) {
             pr.zmarch = true;
@@ -516,9 +521,41 @@ bool make_fixed_div(real_t b, sw::FixedDiv &d)
 // Tile configurations of k_step3d_vec as (rows per wavefront WR, rows per
 // thread RPT, wavefronts per block NW).  Radius 4 (space order 8) carries
 // every configuration; other radii carry 0, 4 and 8.  SIMWAVE_HIP_TILE picks
-// one; otherwise default_tile(R).
+// one; otherwise default_tile(R).  Measured per launch on one MI355X
+// (float32): radius 4 is fastest with 8 rows x 64 points and 8-plane z
+// chunks; radii 5-10 with 8 rows x 32 points and 4-plane chunks (10-13%
+// ahead of 4-wavefront tiles at 312^3-376^3).  Float64 keeps its measured
+// radius-4 setting (12-plane chunks) everywhere.
 constexpr int kTileShapes = 17;
-constexpr int default_tile(int R) { return R == 4 ? 10 : R < 4 ? 0 : 4; }
+constexpr int default_tile(int R)
+{
+    return R == 4 ? 10 : R < 4 ? 0 : (sizeof(real_t) == 4 ? 8 : 4);
+}
+constexpr int default_tile_zlen(int R)
+{
+    return sizeof(real_t) == 4 ? (R == 4 ? 8 : R > 4 ? 4 : 12) : 12;
+}
+
+// Whether the z-marching one-point-per-thread kernel beats the tiled one
+// for this grid.  Its 7 wavefronts per SIMD hide latency better while the
+// grid holds too few tiles to keep the tiled kernel's 2 per SIMD busy; its
+// neighbour reads through the caches lose once there are plenty at radius 4.
+// Measured per launch (float32, cubes): radius 1-3 at every size up to
+// 466^3 (1.2-1.9x faster than any tile configuration); radius 4 up to
+// ~50 M interior points (1.3x at 22 M, even at 38 M, tiled 7% ahead at 60 M
+// and 9% at 100 M); radius 5+ never (its register queue is 2R+1 deep).
+// Float64: the tiled kernel (z-marching 26% slower at 22 M, radius 4).
+bool prefer_zmarch(const Prepared &pr)
+{
+    if (sizeof(real_t) != 4)
+        return false;
+    const Params &P = pr.P;
+    const int R = pr.r;
+    const long long interior = (long long)(P.nz - 2 * R) * (P.nx - 2 * R) * (P.ny - 2 * R);
+    if (R <= 3)
+        return true;
+    return R == 4 && interior < 50ll * 1000 * 1000;
+}
 
 template <int R, int WR, int RPT, int NW>
 void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un,
@@ -640,13 +677,14 @@ void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
     T.ncols = T.gk * T.gj;
     T.order = env_int("SIMWAVE_HIP_ORDER", 1) == 1 ? 1 : 0;
     T.chunk = (T.ncols + 7) / 8;
-    // z chunks of about 12 planes: short tiles keep many independent blocks
-    // in every phase (prologue, march) on each CU, which overlaps their
-    // loads and arithmetic better than long marches, at 2R/zlen extra
-    // centre-plane reads
+    // short z chunks (default_tile_zlen planes): short tiles keep many
+    // independent blocks in every phase (prologue, march) on each CU, which
+    // overlaps their loads and arithmetic better than long marches, at
+    // 2R/zlen extra centre-plane reads
     int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
     if (zlen <= 0) {
-        const int chunks = std::max(1, (Iz + 11) / 12);
+        const int target = default_tile_zlen(R);
+        const int chunks = std::max(1, (Iz + target - 1) / target);
         zlen = (Iz + chunks - 1) / chunks;
     }
     zlen = std::max(1, std::min({zlen, Iz, 64}));   // one 64-bit damping mask per tile
@@ -719,10 +757,13 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
 {
     const int Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
     const int gx = (Ix + kTB - 1) / kTB;
+    // about 2048 blocks, down to one row each: a block's rows form one
+    // dependent load chain, so small grids run fastest with the most blocks
+    // (517^2 so4: 7.7 us per step at 1 row against 14.4 us at 9 rows;
+    // 1136 x 3136 so16: 57 us at 8 rows, within 3% of the best)
     int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
     if (zlen <= 0) {
-        int nzc = std::max(1, (1024 + gx - 1) / gx);
-        nzc = std::min(nzc, std::max(1, Iz / 8));
+        const int nzc = std::min(std::max(1, (2048 + gx - 1) / gx), std::max(1, Iz));
         zlen = (Iz + nzc - 1) / nzc;
     }
     const int gz = (Iz + zlen - 1) / zlen;
@@ -1425,7 +1466,7 @@ double run_host(const Args &a)
         t_down = now_s() - tl1;
         g_timings[6] = pr.fused ? 1.0 : 0.0;
 #if SW_DIM == 3
-        g_timings[7] = pr.tiled ? 1.0 : 0.0;
+        g_timings[7] = pr.tiled ? 1.0 : pr.zmarch ? 2.0 : 0.0;
 #endif
     }
     const double total = now_s() - t0;
@@ -1605,7 +1646,7 @@ size_t run_plan(Plan &p, size_t count, hipStream_t st)
     p.done = i1;
     g_timings[6] = p.pr.fused ? 1.0 : 0.0;
 #if SW_DIM == 3
-    g_timings[7] = p.pr.tiled ? 1.0 : 0.0;
+    g_timings[7] = p.pr.tiled ? 1.0 : p.pr.zmarch ? 2.0 : 0.0;
 #endif
     return p.slots.size() - p.done;
 }

@@ -253,6 +253,53 @@ def test_parity_zmarch_kernel(gpu, name):
         os.environ.pop("SIMWAVE_HIP_ZMARCH", None)
 
 
+def _step_kernel_of_last_run(prec):
+    """The 3D constant-density library's record of its last call on this
+    thread: 1 tiled kernel, 2 z-marching kernel, 0 other."""
+    import ctypes
+    import simwave_amd as sw
+    lib = ctypes.CDLL(sw.build.library_path(3, "cd", prec))
+    out = (ctypes.c_double * 8)()
+    lib.simwave_hip_last_timings.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    lib.simwave_hip_last_timings.restype = ctypes.c_int
+    assert lib.simwave_hip_last_timings(out, 8) == 8
+    return int(out[7])
+
+
+# (case, tile forced, expected 3D step kernel): float32 grids of radius
+# <= 3, and small ones of radius 4, take the z-marching kernel; radius > 4
+# and float64 the tiled one; a forced tile configuration always the tiled
+# one.
+KERNEL_CHOICE = [("3d_cd_so8", None, 2), ("3d_cd_so2", None, 2), ("3d_wide_so6", None, 2),
+                 ("3d_cd_so20", None, 1),
+                 ("3d_cd_f64_so8", None, 1), ("3d_cd_so8", "10", 1)]
+
+
+@pytest.mark.parametrize("name,tile,expect", KERNEL_CHOICE)
+def test_default_step_kernel_choice(gpu, name, tile, expect):
+    print("case", name, "tile", tile, flush=True)
+    if tile is not None:
+        os.environ["SIMWAVE_HIP_TILE"] = tile
+    try:
+        _compare(name, CASES[name], generic=False)
+        prec = "f64" if CASES[name].get("dtype") == F64 else "f32"
+        assert _step_kernel_of_last_run(prec) == expect
+    finally:
+        os.environ.pop("SIMWAVE_HIP_TILE", None)
+
+
+# 2D: small grids default to one-row blocks; longer z marches stay covered.
+@pytest.mark.parametrize("zlen", ["3", "17"])
+@pytest.mark.parametrize("name", ["2d_cd_so8", "2d_vd_so16", "2d_bc_mixed", "2d_src_halo"])
+def test_parity_2d_z_marches(gpu, name, zlen):
+    print("case", name, "zlen", zlen, flush=True)
+    os.environ["SIMWAVE_HIP_ZLEN"] = zlen
+    try:
+        _compare(name, CASES[name], generic=False)
+    finally:
+        os.environ.pop("SIMWAVE_HIP_ZLEN", None)
+
+
 def test_parity_planes_beyond_tiled_offsets(gpu):
     """1920 x 1920 float64 planes (29.5 MB): the tiled kernel's offsets do
     not fit, the z-marching kernel runs."""
