@@ -475,6 +475,26 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
             pr.zmarch_zlen = std::max(1, std::min(env_int("SIMWAVE_HIP_ZLEN", 8), 256));
         }
     }
+#elif SW_DIM == 3
+    // variable density: the z-marching kernel where it is the faster one
+    // (prefer_zmarch) or when asked for; the LDS-tiled k_step3d otherwise.
+    // Its z chunks: 16 planes, halved (down to 4) while the launch holds
+    // fewer than 2048 blocks (448^3 so8: 854 us at 16 planes, 894 at 8;
+    // 176^3 so8: 65 us at 4 planes, 71 at 16)
+    if (fused && !env_flag("SIMWAVE_HIP_UNTILED") &&
+        (env_flag("SIMWAVE_HIP_ZMARCH") || prefer_zmarch(pr))) {
+        pr.zmarch = true;
+        int zl = env_int("SIMWAVE_HIP_ZLEN", 0);
+        if (zl <= 0) {
+            const long long cols = (long long)((P.ny - 2 * r + 63) / 64) *
+                                   ((P.nx - 2 * r + sw::kZLines - 1) / sw::kZLines);
+            const int iz = P.nz - 2 * r;
+            zl = 16;
+            while (zl > 4 && cols * ((iz + zl - 1) / zl) < 2048)
+                zl /= 2;
+        }
+        pr.zmarch_zlen = std::max(1, std::min(zl, 256));
+    }
 #endif
     return pr;
 }
@@ -545,10 +565,16 @@ constexpr int default_tile_zlen(int R)
 // ~50 M interior points (1.3x at 22 M, even at 38 M, tiled 7% ahead at 60 M
 // and 9% at 100 M); radius 5+ never (its register queue is 2R+1 deep).
 // Float64: the tiled kernel (z-marching 26% slower at 22 M, radius 4).
+// Variable density (float32): at every radius and size measured, 1.2-1.8x
+// faster than the LDS-tiled k_step3d (176^3-448^3, orders 2-16); float64
+// keeps k_step3d, whose doubles the z-marching register queues would make
+// too large.
 bool prefer_zmarch(const Prepared &pr)
 {
     if (sizeof(real_t) != 4)
         return false;
+    if (SW_VD)
+        return true;
     const Params &P = pr.P;
     const int R = pr.r;
     const long long interior = (long long)(P.nz - 2 * R) * (P.nx - 2 * R) * (P.ny - 2 * R);
@@ -852,11 +878,10 @@ struct RecStream {
 };
 
 #if SW_DIM == 3
-// One step of k_step3d_z over the interior (constant density).
+// One step of k_step3d_z over the interior.
 void zmarch_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un, int n,
                    hipStream_t st)
 {
-#if !SW_VD
     const Params &P = pr.P;
     const int R = pr.r, zl = pr.zmarch_zlen;
     const dim3 grid((unsigned)((P.ny - 2 * R + 63) / 64),
@@ -874,10 +899,6 @@ void zmarch_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_
     default:
         fail("stencil radius %d out of range", R);
     }
-#else
-    (void)pr; (void)uc; (void)up; (void)un; (void)n; (void)st;
-    fail("the z-marching kernel covers constant density only");
-#endif
 }
 #endif
 

@@ -437,15 +437,19 @@ k_step3d(Params P, const real_t *__restrict__ uc, const real_t *up,
 }
 #endif
 
-#if SW_DIM == 3 && !SW_VD
-// Fused 3D constant-density step, one point per thread: a thread owns a
-// (j, k) column of a z chunk, keeps u's z neighbours in a register queue and
-// loads every operand of plane i+1 (queue head, x/y neighbours, u_prev, q,
+#if SW_DIM == 3
+// Fused 3D step, one point per thread: a thread owns a (j, k) column of a z
+// chunk, keeps u's (and rho's) z neighbours in a register queue and loads
+// every operand of plane i+1 (queue heads, x/y neighbours, u_prev, q,
 // damping term) while plane i is computed; x/y neighbours come through the
 // caches.  Same operations in the same order as k_step3d.  Plain 64-bit
-// addressing: it serves grids whose planes are too large for the tiled
-// kernel's 32-bit buffer offsets (about 13% slower than the tiled kernel at
-// 472^3).
+// addressing, so it also serves grids whose planes are too large for the
+// tiled kernel's 32-bit buffer offsets.  Its small per-thread state (72
+// VGPRs at radius 4, constant density) keeps 7 wavefronts per SIMD, which
+// hides latency better than the tiled kernel wherever the grid holds too
+// few tiles to keep that kernel's 2 per SIMD busy.  Variable density: the
+// fused path requires nx == ny (wave.c's x-axis first derivatives step by
+// nx), so the x neighbours at m * sx are the reference's.
 constexpr int kZLines = 8;
 
 template <int R>
@@ -462,12 +466,24 @@ k_step3d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ u
     const long long sz = P.sz, sx = P.sx;
     const long long col = (long long)j * sx + k;
     real_t qz[2 * R + 1];
+#if SW_VD
+    const real_t *__restrict__ rho = P.rho;
+    real_t rz[2 * R + 1];
+#endif
 #pragma unroll
-    for (int m = 0; m < 2 * R; m++)
+    for (int m = 0; m < 2 * R; m++) {
         qz[m + 1] = uc[(long long)(z0 - R + m) * sz + col];
+#if SW_VD
+        rz[m + 1] = rho[(long long)(z0 - R + m) * sz + col];
+#endif
+    }
     struct Ops {
         real_t head, up, q, e;
         real_t yn[2 * R], xn[2 * R];
+#if SW_VD
+        real_t rhead;
+        real_t ryn[2 * R], rxn[2 * R];
+#endif
     };
     auto load = [&](int i, Ops &o) {
         const long long p = (long long)i * sz + col;
@@ -479,6 +495,16 @@ k_step3d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ u
             o.xn[2 * m - 2] = uc[p + m * sx];
             o.xn[2 * m - 1] = uc[p - m * sx];
         }
+#if SW_VD
+        o.rhead = rho[p + R * sz];
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            o.ryn[2 * m - 2] = rho[p + m];
+            o.ryn[2 * m - 1] = rho[p - m];
+            o.rxn[2 * m - 2] = rho[p + m * sx];
+            o.rxn[2 * m - 1] = rho[p - m * sx];
+        }
+#endif
         o.up = up[p];
         o.q = P.q[p];
         o.e = P.e[p];
@@ -489,22 +515,50 @@ k_step3d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ u
         if (i + 1 < z1)
             load(i + 1, nxt);
 #pragma unroll
-        for (int m = 0; m < 2 * R; m++)
+        for (int m = 0; m < 2 * R; m++) {
             qz[m] = qz[m + 1];
+#if SW_VD
+            rz[m] = rz[m + 1];
+#endif
+        }
         qz[2 * R] = cur.head;
+#if SW_VD
+        rz[2 * R] = cur.rhead;
+#endif
         const long long p = (long long)i * sz + col;
         const real_t c = qz[R];
         real_t acc_c = P.c2[0] * c;
         real_t acc_m = P.c2[0] * c;
         real_t acc_z = P.c2[0] * c;
+#if SW_VD
+        real_t du_c = 0, dr_c = 0, du_m = 0, dr_m = 0, du_z = 0, dr_z = 0;
+#endif
 #pragma unroll
         for (int m = 1; m <= R; m++) {
-            acc_c += P.c2[m] * (cur.yn[2 * m - 2] + cur.yn[2 * m - 1]);
-            acc_m += P.c2[m] * (cur.xn[2 * m - 2] + cur.xn[2 * m - 1]);
+            const real_t cp = cur.yn[2 * m - 2], cm = cur.yn[2 * m - 1];
+            const real_t mp = cur.xn[2 * m - 2], mm = cur.xn[2 * m - 1];
+            acc_c += P.c2[m] * (cp + cm);
+            acc_m += P.c2[m] * (mp + mm);
             acc_z += P.c2[m] * (qz[R + m] + qz[R - m]);
+#if SW_VD
+            du_c += P.c1[m] * (cp - cm);
+            dr_c += P.c1[m] * (cur.ryn[2 * m - 2] - cur.ryn[2 * m - 1]);
+            du_m += P.c1[m] * (mp - mm);
+            dr_m += P.c1[m] * (cur.rxn[2 * m - 2] - cur.rxn[2 * m - 1]);
+            du_z += P.c1[m] * (qz[R + m] - qz[R - m]);
+            dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+#endif
         }
         real_t value = 0;
         value += acc_c / P.dy2 + acc_m / P.dx2 + acc_z / P.dz2;
+#if SW_VD
+        {
+            const real_t t_c = (du_c * dr_c) / (4 * P.dy2);
+            const real_t t_m = (du_m * dr_m) / (4 * P.dx2);
+            const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
+            value -= (t_c + t_m + t_z) / rz[R];
+        }
+#endif
         real_t v = leapfrog(value, cur.q, cur.e, c, cur.up);
         if (P.nsrc)
             v = add_sources(P, n, i, j, k, p, v);

@@ -52,6 +52,12 @@ CASES = {
     "3d_vd_so8": dict(dim=3, n=28, space_order=8, density=True, tf=0.08),
     "3d_vd_quirk": dict(dim=3, shape=(26, 30, 24), space_order=4, density=True,
                         tf=0.08),
+    "3d_vd_bc": dict(dim=3, n=24, density=True, damping=0.0,
+                     bc=("null_neumann", "null_dirichlet", "null_neumann", "none",
+                         "null_dirichlet", "null_neumann"),
+                     src=[(0.0, 100.0, 120.0), (8.0, 5.0, 200.0)], multi=True, tf=0.06),
+    "3d_vd_so16_wide": dict(dim=3, shape=(30, 90, 90), space_order=16, density=True,
+                            damping=30.0, tf=0.03),
     "3d_cd_f64_so8": dict(dim=3, n=26, space_order=8, dtype=F64, tf=0.08),
     "3d_vd_f64_so4": dict(dim=3, n=26, space_order=4, density=True, dtype=F64,
                           tf=0.08),
@@ -240,7 +246,8 @@ def test_host_entry_keeps_initial_halo_values(gpu, name):
 # representative cases, and taken by itself where planes are too large for
 # the tiled kernel's 32-bit buffer offsets.
 ZMARCH = ["3d_cd_so2", "3d_cd_so8", "3d_cd_so20", "3d_cd_f64_so8", "3d_bc_neumann",
-          "3d_bc_mixed", "3d_src_halo", "3d_wide_bc", "3d_deep_so8"]
+          "3d_bc_mixed", "3d_src_halo", "3d_wide_bc", "3d_deep_so8",
+          "3d_vd_so4", "3d_vd_so8", "3d_vd_f64_so4", "3d_vd_bc", "3d_vd_so16_wide"]
 
 
 @pytest.mark.parametrize("name", ZMARCH)
@@ -253,12 +260,12 @@ def test_parity_zmarch_kernel(gpu, name):
         os.environ.pop("SIMWAVE_HIP_ZMARCH", None)
 
 
-def _step_kernel_of_last_run(prec):
-    """The 3D constant-density library's record of its last call on this
-    thread: 1 tiled kernel, 2 z-marching kernel, 0 other."""
+def _step_kernel_of_last_run(dens, prec):
+    """The 3D library's record of its last call on this thread: 1 tiled
+    kernel, 2 z-marching kernel, 0 other (k_step3d for variable density)."""
     import ctypes
     import simwave_amd as sw
-    lib = ctypes.CDLL(sw.build.library_path(3, "cd", prec))
+    lib = ctypes.CDLL(sw.build.library_path(3, dens, prec))
     out = (ctypes.c_double * 8)()
     lib.simwave_hip_last_timings.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     lib.simwave_hip_last_timings.restype = ctypes.c_int
@@ -266,13 +273,16 @@ def _step_kernel_of_last_run(prec):
     return int(out[7])
 
 
-# (case, tile forced, expected 3D step kernel): float32 grids of radius
-# <= 3, and small ones of radius 4, take the z-marching kernel; radius > 4
-# and float64 the tiled one; a forced tile configuration always the tiled
-# one.
+# (case, tile forced, expected 3D step kernel): constant-density float32
+# grids of radius <= 3, and small ones of radius 4, take the z-marching
+# kernel; radius > 4 and float64 the tiled one; a forced tile configuration
+# always the tiled one.  Variable density: z-marching for float32,
+# k_step3d for float64.
 KERNEL_CHOICE = [("3d_cd_so8", None, 2), ("3d_cd_so2", None, 2), ("3d_wide_so6", None, 2),
                  ("3d_cd_so20", None, 1),
-                 ("3d_cd_f64_so8", None, 1), ("3d_cd_so8", "10", 1)]
+                 ("3d_cd_f64_so8", None, 1), ("3d_cd_so8", "10", 1),
+                 ("3d_vd_so8", None, 2), ("3d_vd_so16_wide", None, 2),
+                 ("3d_vd_f64_so4", None, 0)]
 
 
 @pytest.mark.parametrize("name,tile,expect", KERNEL_CHOICE)
@@ -283,9 +293,26 @@ def test_default_step_kernel_choice(gpu, name, tile, expect):
     try:
         _compare(name, CASES[name], generic=False)
         prec = "f64" if CASES[name].get("dtype") == F64 else "f32"
-        assert _step_kernel_of_last_run(prec) == expect
+        dens = "vd" if CASES[name].get("density") else "cd"
+        assert _step_kernel_of_last_run(dens, prec) == expect
     finally:
         os.environ.pop("SIMWAVE_HIP_TILE", None)
+
+
+# The LDS-tiled k_step3d (SIMWAVE_HIP_UNTILED): the float64 variable-density
+# default, forced here for float32 and constant density too.
+UNTILED = ["3d_vd_so4", "3d_vd_so8", "3d_vd_bc", "3d_vd_so16_wide", "3d_cd_so8",
+           "3d_bc_mixed", "3d_src_halo"]
+
+
+@pytest.mark.parametrize("name", UNTILED)
+def test_parity_lds_tiled_kernel(gpu, name):
+    print("case", name, "(k_step3d)", flush=True)
+    os.environ["SIMWAVE_HIP_UNTILED"] = "1"
+    try:
+        _compare(name, CASES[name], generic=False)
+    finally:
+        os.environ.pop("SIMWAVE_HIP_UNTILED", None)
 
 
 # 2D: small grids default to one-row blocks; longer z marches stay covered.
