@@ -274,8 +274,9 @@ struct Prepared {
     int r = 0;
     long long npts = 0;
 #if SW_DIM == 3
-    bool zmarch = false;         // use k_step3d_z (SIMWAVE_HIP_ZMARCH=1)
+    bool zmarch = false;         // use k_step3d_z
     int zmarch_zlen = 16;
+    sw::LapDiv lapdiv{};         // its fixed-divisor quotients
     bool tiled = false;          // use k_step3d_vec
     void (*tiled_launch)(const struct Prepared &, const real_t *, const real_t *,
                          real_t *, int, hipStream_t) = nullptr;
@@ -291,6 +292,7 @@ struct Prepared {
 #if SW_DIM == 3
 void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st);
 bool prefer_zmarch(const Prepared &pr);
+void set_lap_div(Prepared &pr);
 #endif
 
 bool env_flag(const char *name)
@@ -473,6 +475,7 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
         if (!pr.tiled) {
             pr.zmarch = true;
             pr.zmarch_zlen = std::max(1, std::min(env_int("SIMWAVE_HIP_ZLEN", 8), 256));
+            set_lap_div(pr);
         }
     }
 #elif SW_DIM == 3
@@ -494,6 +497,7 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
                 zl /= 2;
         }
         pr.zmarch_zlen = std::max(1, std::min(zl, 256));
+        set_lap_div(pr);
     }
 #endif
     return pr;
@@ -536,6 +540,19 @@ bool make_fixed_div(real_t b, sw::FixedDiv &d)
     d.hi = (real_t)std::ldexp(1.0, emax - 2 + std::min(0, eb));
     return true;
 }
+
+#if SW_DIM == 3
+// Fixed divisors for k_step3d_z's Laplacian quotients (IEEE ones when a
+// divisor has none or SIMWAVE_HIP_IEEE_DIV is set).
+void set_lap_div(Prepared &pr)
+{
+    sw::LapDiv &D = pr.lapdiv;
+    D.ok = !env_flag("SIMWAVE_HIP_IEEE_DIV") && make_fixed_div(pr.P.dy2, D.dy) &&
+           make_fixed_div(pr.P.dx2, D.dx) && make_fixed_div(pr.P.dz2, D.dz);
+    D.lo = std::max({D.dy.lo, D.dx.lo, D.dz.lo});
+    D.hi = std::min({D.dy.hi, D.dx.hi, D.dz.hi});
+}
+#endif
 
 #if SW_DIM == 3
 // Tile configurations of k_step3d_vec as (rows per wavefront WR, rows per
@@ -891,7 +908,7 @@ void zmarch_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_
     switch (R) {
 #define SW_Z_CASE(RR)                                                                   \
     case RR:                                                                            \
-        hipLaunchKernelGGL(sw::k_step3d_z<RR>, grid, block, 0, st, P, uc, up, un, n, zl); \
+        hipLaunchKernelGGL(sw::k_step3d_z<RR>, grid, block, 0, st, P, pr.lapdiv, uc, up, un, n, zl); \
         return;
     SW_Z_CASE(1) SW_Z_CASE(2) SW_Z_CASE(3) SW_Z_CASE(4) SW_Z_CASE(5)
     SW_Z_CASE(6) SW_Z_CASE(7) SW_Z_CASE(8) SW_Z_CASE(9) SW_Z_CASE(10)

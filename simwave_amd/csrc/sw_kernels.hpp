@@ -121,6 +121,30 @@ __device__ __forceinline__ real_t fixed_div(real_t a, const FixedDiv &d)
 #endif
 }
 
+// The divisors of the Laplacian's three axis sums (dy2, dx2, dz2) for
+// fixed_div, and the numerator range safe for all three; ok == 0 keeps the
+// IEEE quotients.
+struct LapDiv {
+    FixedDiv dy, dx, dz;
+    real_t lo, hi;
+    int ok;
+};
+
+// acc_y / dy2 + acc_x / dx2 + acc_z / dz2 (cd-3d wave.c:186), every quotient
+// correctly rounded: fixed-divisor quotients when all active lanes of the
+// wavefront have their three numerators in the safe range (or all zero),
+// IEEE quotients otherwise.  Same bits either way.
+__device__ __forceinline__ real_t lap_quotients(const Params &P, const LapDiv &D,
+                                                real_t ay, real_t ax, real_t az)
+{
+    const real_t mx = fmax(fmax(fabs(ay), fabs(ax)), fabs(az));
+    const real_t mn = fmin(fmin(fabs(ay), fabs(ax)), fabs(az));
+    const bool ok = (D.ok != 0) & (((mn >= D.lo) & (mx <= D.hi)) | (mx == (real_t)0));
+    if (__all(ok))
+        return fixed_div(ay, D.dy) + fixed_div(ax, D.dx) + fixed_div(az, D.dz);
+    return ay / P.dy2 + ax / P.dx2 + az / P.dz2;
+}
+
 // Source contribution at one point: dt2/s * kws * w / den.
 // Reference: cd-3d wave.c:272-277.
 __device__ __forceinline__ real_t source_term(real_t q, real_t e, real_t kws,
@@ -454,7 +478,7 @@ constexpr int kZLines = 8;
 
 template <int R>
 __global__ void __launch_bounds__(64 * kZLines)
-k_step3d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ up,
+k_step3d_z(Params P, LapDiv D, const real_t *__restrict__ uc, const real_t *__restrict__ up,
            real_t *__restrict__ un, int n, int zlen)
 {
     const int k = R + blockIdx.x * 64 + threadIdx.x;
@@ -550,7 +574,7 @@ k_step3d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ u
 #endif
         }
         real_t value = 0;
-        value += acc_c / P.dy2 + acc_m / P.dx2 + acc_z / P.dz2;
+        value += lap_quotients(P, D, acc_c, acc_m, acc_z);
 #if SW_VD
         {
             const real_t t_c = (du_c * dr_c) / (4 * P.dy2);
