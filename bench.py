@@ -383,7 +383,7 @@ def run_serving(args):
                                                steps=(20, 20))[0]]
     log("serving: warm-up (%d small shots)" % len(warm))
     with ShotServer(hbm_budget=args.hbm_budget * 1e9, max_active=args.max_active,
-                    chunk_steps=args.chunk_steps) as srv:
+                    chunk_steps=args.chunk_steps, plan_flags=args.plan_flags) as srv:
         for f in [srv.submit(s) for s in warm]:
             f.result(timeout=120)
     log("serving: warm-up done")
@@ -391,7 +391,7 @@ def run_serving(args):
     gaps = rng.exponential(1.0 / args.rate, size=len(solvers))
     torch.cuda.synchronize()
     with ShotServer(hbm_budget=args.hbm_budget * 1e9, max_active=args.max_active,
-                    chunk_steps=args.chunk_steps) as srv:
+                    chunk_steps=args.chunk_steps, plan_flags=args.plan_flags) as srv:
         start = time.perf_counter()
         futs = []
         due = start
@@ -444,6 +444,7 @@ def run_serving(args):
         "config": {"workload": "serving", "requests": len(solvers), "seed": args.seed,
                    "arrival_rate_hz": args.rate, "hbm_budget_gb": args.hbm_budget,
                    "max_active": args.max_active, "chunk_steps": args.chunk_steps,
+                   "plan_flags": args.plan_flags,
                    "description": serving_requests.__doc__.split("\n")[0]},
         "shot_latency_ms": {"p50": round(summ["latency_p50_s"] * 1e3, 2),
                             "p99": round(summ["latency_p99_s"] * 1e3, 2),
@@ -452,6 +453,8 @@ def run_serving(args):
                     "queue_p50_ms": round(summ["queue_p50_s"] * 1e3, 2),
                     "max_active_seen": summ["max_active"],
                     "prep_mean_ms": round(summ["prep_mean_s"] * 1e3, 2),
+                    "prep_build_mean_ms": round(summ["prep_build_mean_s"] * 1e3, 2),
+                    "prep_plan_mean_ms": round(summ["prep_plan_mean_s"] * 1e3, 2),
                     "device_mean_ms": round(summ["device_mean_s"] * 1e3, 2),
                     "fetch_mean_ms": round(summ["fetch_mean_s"] * 1e3, 2),
                     "plain_path": "Solver.forward(), one shot at a time",
@@ -574,6 +577,8 @@ def main():
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving budget, GB")
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--chunk-steps", type=int, default=64)
+    ap.add_argument("--plan-flags", type=int, default=None,
+                    help="serving: plan flags (default: the server's, graph replay)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (counter scale)")

@@ -83,8 +83,12 @@ class ServeStats:
             "queue_p50_s": percentile([r["admitted"] - r["arrival"] for r in self.records], 50),
             "max_active": max(r["active_at_admission"] for r in self.records),
             # mean per-request phases on its worker: argument preparation
-            # and uploads, enqueue-to-completion on the device, result fetch
+            # and uploads (split into building the device-resident shot and
+            # creating its plan), enqueue-to-completion on the device, result
+            # fetch
             "prep_mean_s": float(np.mean([r.get("prep", np.nan) for r in self.records])),
+            "prep_build_mean_s": float(np.mean([r.get("build", np.nan) for r in self.records])),
+            "prep_plan_mean_s": float(np.mean([r.get("plan", np.nan) for r in self.records])),
             "device_mean_s": float(np.mean([r.get("device", np.nan) for r in self.records])),
             "fetch_mean_s": float(np.mean([r.get("fetch", np.nan) for r in self.records])),
         }
@@ -232,6 +236,7 @@ class ShotServer:
             t0 = time.perf_counter()
             with ctx:
                 shot = self._shot_factory(req.solver)
+            tb = time.perf_counter()
             shot.plan(stream, flags=self.plan_flags)
             t1 = time.perf_counter()
             left = shot.steps
@@ -244,7 +249,8 @@ class ShotServer:
             done = time.perf_counter()
             self.stats.add(arrival=req.arrival, admitted=req.admitted, done=done,
                            updates=shot.updates_per_shot, active_at_admission=active_now,
-                           prep=t1 - t0, device=t2 - t1, fetch=done - t2)
+                           prep=t1 - t0, build=tb - t0, plan=t1 - tb, device=t2 - t1,
+                           fetch=done - t2)
             req.future.set_result(result)
         except Exception as exc:
             if shot is not None:
