@@ -72,6 +72,10 @@ def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=Fa
                          device=dev.device)
     if stream is None:
         stream = torch.cuda.current_stream(dev.device)
+    # the zeroed outputs above are queued on the current stream, the shot's
+    # uploads on the stream it was built on
+    stream.wait_stream(torch.cuda.current_stream(dev.device))
+    stream.wait_event(dev._ready)
     secs = fn(*dev._args, ctypes.c_void_p(rec.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
               ctypes.c_void_p(gw.data_ptr() if gw is not None else 0),
               ctypes.c_size_t(int(checkpoint_every)), ctypes.c_void_p(stream.cuda_stream),

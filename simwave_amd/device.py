@@ -114,15 +114,24 @@ class DeviceForward:
         self._plan_destroy.argtypes = [ctypes.c_void_p]
         self._plan = None
         self._argtypes = argtypes
+        # the uploads and gathers above are queued on the current stream;
+        # work on any other stream waits for them (device side)
+        self._ready = torch.cuda.Event()
+        self._ready.record(torch.cuda.current_stream(self.device))
+
+    def _upload(self, array):
+        """Host array -> device tensor (a plain copy; staging through
+        freshly pinned memory made 16 concurrent shots slower)."""
+        return self._torch.from_numpy(np.ascontiguousarray(array)).to(self.device)
 
     def _gather(self, small, index):
         """small[np.ix_(*index)] built on the device: one upload of `small`,
         then an index_select per axis (copies only, so the values are the
         host's, bit for bit)."""
         torch = self._torch
-        t = torch.from_numpy(np.ascontiguousarray(small)).to(self.device)
+        t = self._upload(small)
         for axis, idx in enumerate(index):
-            t = torch.index_select(t, axis, torch.from_numpy(idx).to(self.device))
+            t = torch.index_select(t, axis, self._upload(idx))
         return t.contiguous()
 
     @property
@@ -145,6 +154,7 @@ class DeviceForward:
             raise RuntimeError("plan already created")
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
+        stream.wait_event(self._ready)
         h = self._plan_create(*self._args, ctypes.c_void_p(stream.cuda_stream),
                               ctypes.c_uint(flags))
         if not h:
@@ -201,6 +211,7 @@ class DeviceForward:
         torch = self._torch
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
+        stream.wait_event(self._ready)
         secs = self._fn(*self._args, ctypes.c_void_p(stream.cuda_stream),
                         ctypes.c_uint(flags))
         if secs < 0:
@@ -230,6 +241,7 @@ class DeviceForward:
         sm, tm = self.solver.space_model, self.solver.time_model
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
+        stream.wait_event(self._ready)
         with torch.cuda.stream(stream):
             u = sm.remove_halo_region(tm.remove_time_halo_region(self.tensors["u_full"]))
             u = u.contiguous()
