@@ -658,8 +658,8 @@ def main():
             kname = "k_stencil_generic"
         elif ndim == 3 and tm[7] == 1.0:
             kname = "k_step3d_vec"
-        elif ndim == 3 and tm[7] == 2.0:
-            kname = "k_step3d_z"
+        elif tm[7] == 2.0:
+            kname = "k_step%dd_z" % ndim
         else:
             kname = "k_step%dd" % ndim
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -917,7 +917,7 @@ int simwave_hip_fixed_div_3d_vd_f64(double b, double *y, double *lo, double *hi)
 const char *simwave_hip_last_error(void);
 /* Phase timings of the last call on this thread: total s, prepare s, time
  * loop s, download s, step-kernel s (flags & 1), step launches, 1 when the
- * fused kernels ran, 3D step kernel (1 tiled, 2 z-marching, 0 other).
+ * fused kernels ran, step kernel (1 3D tiled, 2 z-marching, 0 other).
  * Writes min(n, 8) values; returns that count. */
 int simwave_hip_last_timings(double *out, int n);
 /* The library's variant, e.g. "3d_cd_f32". */

@@ -794,6 +794,13 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
 #else
 constexpr int kTB = 256;
 
+// Whether 2D steps run k_step2d_z (see launch_fused).
+bool zmarch_2d()
+{
+    return env_flag("SIMWAVE_HIP_ZMARCH") ||
+           (sizeof(real_t) == 4 && !env_flag("SIMWAVE_HIP_UNTILED"));
+}
+
 template <int R>
 void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *un,
                   int n, hipStream_t st)
@@ -810,6 +817,16 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
         zlen = (Iz + nzc - 1) / nzc;
     }
     const int gz = (Iz + zlen - 1) / zlen;
+    // float32: the z-marching kernel (0-27% faster than the LDS row kernel
+    // on 2D grids of 304^2 to 1136 x 3136, orders 2-20, CD and VD: 41.7 vs
+    // 57.0 us per step at 1136 x 3136 so16 VD); float64 keeps k_step2d
+    // (SIMWAVE_HIP_UNTILED forces it, SIMWAVE_HIP_ZMARCH the z-marching one)
+    if (zmarch_2d()) {
+        static_assert(sw::kZ2Block == kTB, "2D launch shapes differ");
+        hipLaunchKernelGGL(sw::k_step2d_z<R>, dim3(gx, gz), dim3(sw::kZ2Block), 0, st,
+                           P, uc, up, un, n, zlen);
+        return;
+    }
     hipLaunchKernelGGL((sw::k_step2d<R, kTB>), dim3(gx, gz), dim3(kTB), 0, st,
                        P, uc, up, un, n, zlen);
 }
@@ -1505,6 +1522,8 @@ double run_host(const Args &a)
         g_timings[6] = pr.fused ? 1.0 : 0.0;
 #if SW_DIM == 3
         g_timings[7] = pr.tiled ? 1.0 : pr.zmarch ? 2.0 : 0.0;
+#else
+        g_timings[7] = pr.fused && zmarch_2d() ? 2.0 : 0.0;
 #endif
     }
     const double total = now_s() - t0;
@@ -1685,6 +1704,8 @@ size_t run_plan(Plan &p, size_t count, hipStream_t st)
     g_timings[6] = p.pr.fused ? 1.0 : 0.0;
 #if SW_DIM == 3
     g_timings[7] = p.pr.tiled ? 1.0 : p.pr.zmarch ? 2.0 : 0.0;
+#else
+    g_timings[7] = p.pr.fused && zmarch_2d() ? 2.0 : 0.0;
 #endif
     return p.slots.size() - p.done;
 }

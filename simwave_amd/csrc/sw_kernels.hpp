@@ -700,6 +700,119 @@ k_step2d(Params P, const real_t *__restrict__ uc, const real_t *up,
 }
 #endif
 
+#if SW_DIM == 2
+// Fused 2D step, one point per thread, the 2D counterpart of k_step3d_z: a
+// thread owns column j of a z chunk, keeps u's (and rho's) z neighbours in
+// a register queue and loads every operand of row i+1 (queue heads, x
+// neighbours, u_prev, q, damping term) while row i is computed; x
+// neighbours come through the caches, so no LDS and no barrier.  Same
+// operations in the same order as k_step2d.
+constexpr int kZ2Block = 256;
+
+template <int R>
+__global__ void __launch_bounds__(kZ2Block)
+k_step2d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ up,
+           real_t *__restrict__ un, int n, int zlen)
+{
+    const int j = R + blockIdx.x * kZ2Block + threadIdx.x;
+    if (j >= P.nx - R)
+        return;
+    const int z0 = R + blockIdx.y * zlen;
+    const int z1 = min(z0 + zlen, P.nz - R);
+    const long long sz = P.sz;
+    real_t qz[2 * R + 1];
+#if SW_VD
+    const real_t *__restrict__ rho = P.rho;
+    real_t rz[2 * R + 1];
+#endif
+#pragma unroll
+    for (int m = 0; m < 2 * R; m++) {
+        qz[m + 1] = uc[(long long)(z0 - R + m) * sz + j];
+#if SW_VD
+        rz[m + 1] = rho[(long long)(z0 - R + m) * sz + j];
+#endif
+    }
+    struct Ops {
+        real_t head, up, q, e;
+        real_t xn[2 * R];
+#if SW_VD
+        real_t rhead;
+        real_t rxn[2 * R];
+#endif
+    };
+    auto load = [&](int i, Ops &o) {
+        const long long p = (long long)i * sz + j;
+        o.head = uc[p + R * sz];
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            o.xn[2 * m - 2] = uc[p + m];
+            o.xn[2 * m - 1] = uc[p - m];
+        }
+#if SW_VD
+        o.rhead = rho[p + R * sz];
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            o.rxn[2 * m - 2] = rho[p + m];
+            o.rxn[2 * m - 1] = rho[p - m];
+        }
+#endif
+        o.up = up[p];
+        o.q = P.q[p];
+        o.e = P.e[p];
+    };
+    Ops cur, nxt;
+    load(z0, cur);
+    for (int i = z0; i < z1; i++) {
+        if (i + 1 < z1)
+            load(i + 1, nxt);
+#pragma unroll
+        for (int m = 0; m < 2 * R; m++) {
+            qz[m] = qz[m + 1];
+#if SW_VD
+            rz[m] = rz[m + 1];
+#endif
+        }
+        qz[2 * R] = cur.head;
+#if SW_VD
+        rz[2 * R] = cur.rhead;
+#endif
+        const long long p = (long long)i * sz + j;
+        const real_t c = qz[R];
+        real_t acc_c = P.c2[0] * c;
+        real_t acc_z = P.c2[0] * c;
+#if SW_VD
+        real_t du_c = 0, dr_c = 0, du_z = 0, dr_z = 0;
+#endif
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            const real_t cp = cur.xn[2 * m - 2], cm = cur.xn[2 * m - 1];
+            acc_c += P.c2[m] * (cp + cm);
+            acc_z += P.c2[m] * (qz[R + m] + qz[R - m]);
+#if SW_VD
+            du_c += P.c1[m] * (cp - cm);
+            dr_c += P.c1[m] * (cur.rxn[2 * m - 2] - cur.rxn[2 * m - 1]);
+            du_z += P.c1[m] * (qz[R + m] - qz[R - m]);
+            dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+#endif
+        }
+        real_t value = 0;
+        value += acc_c / P.dx2 + acc_z / P.dz2;
+#if SW_VD
+        {
+            const real_t t_c = (du_c * dr_c) / (4 * P.dx2);
+            const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
+            value -= (t_c + t_z) / rz[R];
+        }
+#endif
+        real_t v = leapfrog(value, cur.q, cur.e, c, cur.up);
+        if (P.nsrc)
+            v = add_sources(P, n, i, j, 0, p, v);
+        store_with_bc(P, un, p, i, j, 0, v);
+        cur = nxt;
+    }
+}
+#endif
+
 // ------------------------------------------------ generic (exact) path
 // Used where the fused kernel's preconditions do not hold (interior thinner
 // than r+1 points, too many sources, or the 3D variable-density x stride

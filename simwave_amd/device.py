@@ -223,7 +223,7 @@ class DeviceForward:
     def timings(self):
         """Phase timings of the last call on this thread:
         [total s, prepare s, loop s, download s, step-kernel s, step launches,
-         fused(1)/generic(0), 3D step kernel: tiled(1)/z-marching(2)/other(0)]."""
+         fused(1)/generic(0), step kernel: 3D tiled(1)/z-marching(2)/other(0)]."""
         out = (ctypes.c_double * 8)()
         fn = self._lib.simwave_hip_last_timings
         fn.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]

@@ -32,3 +32,26 @@ def test_device_arrays_equal_host_arrays(name):
         got = t.cpu().numpy()
         assert got.shape == ref.shape and got.dtype == ref.dtype, key
         assert np.array_equal(got, ref), key
+
+
+@pytest.mark.parametrize("name", ["3d_cd", "2d_vd_f64"])
+def test_shot_built_on_one_stream_runs_on_another(name):
+    """The shot's uploads and gathers are queued on the stream it is built
+    on; a plan, its chunks and the results fetched on another stream see
+    them (the shot's ready event), and give the same arrays as one stream."""
+    import torch
+    print("cross-stream shot", name, flush=True)
+    solver = sw.Solver(**cases.make_case(**CASES[name])())
+    ref = DeviceForward(solver)
+    ref.run()
+    u_ref, rec_ref = ref.results()
+    build, other = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(build):
+        dev = DeviceForward(solver)
+    dev.plan(other)
+    left = dev.steps
+    while left > 0:
+        left = dev.run_steps(7, other)
+    u, rec = dev.results(other)
+    dev.release_plan()
+    assert np.array_equal(u, u_ref) and np.array_equal(rec, rec_ref)

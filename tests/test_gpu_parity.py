@@ -247,7 +247,10 @@ def test_host_entry_keeps_initial_halo_values(gpu, name):
 # the tiled kernel's 32-bit buffer offsets.
 ZMARCH = ["3d_cd_so2", "3d_cd_so8", "3d_cd_so20", "3d_cd_f64_so8", "3d_bc_neumann",
           "3d_bc_mixed", "3d_src_halo", "3d_wide_bc", "3d_deep_so8",
-          "3d_vd_so4", "3d_vd_so8", "3d_vd_f64_so4", "3d_vd_bc", "3d_vd_so16_wide"]
+          "3d_vd_so4", "3d_vd_so8", "3d_vd_f64_so4", "3d_vd_bc", "3d_vd_so16_wide",
+          "2d_cd_so2", "2d_cd_so8", "2d_cd_so20", "2d_vd_so4", "2d_vd_so16", "2d_cd_f64_so8",
+          "2d_vd_f64_so8", "2d_bc_mixed", "2d_bc_neumann", "2d_src_halo", "2d_src_overlap",
+          "2d_stride3"]
 
 
 @pytest.mark.parametrize("name", ZMARCH)
@@ -260,12 +263,12 @@ def test_parity_zmarch_kernel(gpu, name):
         os.environ.pop("SIMWAVE_HIP_ZMARCH", None)
 
 
-def _step_kernel_of_last_run(dens, prec):
-    """The 3D library's record of its last call on this thread: 1 tiled
-    kernel, 2 z-marching kernel, 0 other (k_step3d for variable density)."""
+def _step_kernel_of_last_run(dim, dens, prec):
+    """The library's record of its last call on this thread: 1 tiled 3D
+    kernel, 2 z-marching kernel, 0 other (k_step3d, k_step2d)."""
     import ctypes
     import simwave_amd as sw
-    lib = ctypes.CDLL(sw.build.library_path(3, dens, prec))
+    lib = ctypes.CDLL(sw.build.library_path(dim, dens, prec))
     out = (ctypes.c_double * 8)()
     lib.simwave_hip_last_timings.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     lib.simwave_hip_last_timings.restype = ctypes.c_int
@@ -277,12 +280,13 @@ def _step_kernel_of_last_run(dens, prec):
 # grids of radius <= 3, and small ones of radius 4, take the z-marching
 # kernel; radius > 4 and float64 the tiled one; a forced tile configuration
 # always the tiled one.  Variable density: z-marching for float32,
-# k_step3d for float64.
+# k_step3d for float64.  2D: z-marching for float32, k_step2d for float64.
 KERNEL_CHOICE = [("3d_cd_so8", None, 2), ("3d_cd_so2", None, 2), ("3d_wide_so6", None, 2),
                  ("3d_cd_so20", None, 1),
                  ("3d_cd_f64_so8", None, 1), ("3d_cd_so8", "10", 1),
                  ("3d_vd_so8", None, 2), ("3d_vd_so16_wide", None, 2),
-                 ("3d_vd_f64_so4", None, 0)]
+                 ("3d_vd_f64_so4", None, 0),
+                 ("2d_cd_so8", None, 2), ("2d_vd_so16", None, 2), ("2d_cd_f64_so8", None, 0)]
 
 
 @pytest.mark.parametrize("name,tile,expect", KERNEL_CHOICE)
@@ -294,15 +298,16 @@ def test_default_step_kernel_choice(gpu, name, tile, expect):
         _compare(name, CASES[name], generic=False)
         prec = "f64" if CASES[name].get("dtype") == F64 else "f32"
         dens = "vd" if CASES[name].get("density") else "cd"
-        assert _step_kernel_of_last_run(dens, prec) == expect
+        assert _step_kernel_of_last_run(CASES[name]["dim"], dens, prec) == expect
     finally:
         os.environ.pop("SIMWAVE_HIP_TILE", None)
 
 
-# The LDS-tiled k_step3d (SIMWAVE_HIP_UNTILED): the float64 variable-density
-# default, forced here for float32 and constant density too.
+# The LDS kernels k_step3d and k_step2d (SIMWAVE_HIP_UNTILED): the float64
+# variable-density 3D and float64 2D defaults, forced here for float32 too.
 UNTILED = ["3d_vd_so4", "3d_vd_so8", "3d_vd_bc", "3d_vd_so16_wide", "3d_cd_so8",
-           "3d_bc_mixed", "3d_src_halo"]
+           "3d_bc_mixed", "3d_src_halo", "2d_cd_so2", "2d_cd_so8", "2d_cd_so20",
+           "2d_vd_so4", "2d_vd_so16", "2d_bc_mixed", "2d_src_halo", "2d_stride3"]
 
 
 @pytest.mark.parametrize("name", UNTILED)
