@@ -129,6 +129,114 @@ k_adj_receivers(Params P, const Box *__restrict__ rbox, const real_t *__restrict
     }
 }
 
+// The same spread as a gather, with no atomics and a fixed order.  Built
+// once per gradient: every (point, receiver) pair of the receiver boxes as
+// an entry, sorted by point (a stable sort of entries made in receiver
+// order), so each touched point sums its receivers' terms in ascending
+// receiver order, as the receiver loop of the transposed step does.
+
+// Entry off[q] + t is point t of receiver q's box: its linear point index
+// (sort key) and its own position (sort value).
+__global__ void __launch_bounds__(256)
+k_adj_rec_entries(Params P, const Box *__restrict__ rbox, int nrec,
+                  const unsigned *__restrict__ off, unsigned *__restrict__ key,
+                  unsigned *__restrict__ idx)
+{
+    const int q = blockIdx.x;
+    if (q >= nrec)
+        return;
+    const Box b = rbox[q];
+    const int cz = b.hi[0] - b.lo[0] + 1, cx = b.hi[1] - b.lo[1] + 1;
+#if SW_DIM == 3
+    const int cy = b.hi[2] - b.lo[2] + 1;
+#else
+    const int cy = 1;
+#endif
+    const unsigned base = off[q];
+    for (int t = threadIdx.x; t < cz * cx * cy; t += blockDim.x) {
+        const int ii = t / (cx * cy), rem = t - ii * cx * cy;
+        const int jj = rem / cy, kk = rem - jj * cy;
+#if SW_DIM == 3
+        const long long p = (long long)(b.lo[0] + ii) * P.sz +
+                            (long long)(b.lo[1] + jj) * P.sx + (b.lo[2] + kk);
+#else
+        (void)kk;
+        const long long p = (long long)(b.lo[0] + ii) * P.sz + (b.lo[1] + jj);
+#endif
+        key[base + t] = (unsigned)p;
+        idx[base + t] = base + t;
+    }
+}
+
+// Sorted entry s (point key[s], original entry idx[s]): its receiver (the q
+// with off[q] <= idx[s] < off[q+1]) and weight.
+__global__ void __launch_bounds__(256)
+k_adj_rec_permute(Params P, const Box *__restrict__ rbox, const real_t *__restrict__ rw,
+                  int nrec, const unsigned *__restrict__ off,
+                  const unsigned *__restrict__ key, const unsigned *__restrict__ idx,
+                  unsigned n, unsigned *__restrict__ qs, real_t *__restrict__ ws)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const unsigned e = idx[s];
+    int lo = 0, hi = nrec;                 // off[lo] <= e < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const long long p = key[s];
+#if SW_DIM == 3
+    const int i = (int)(p / P.sz);
+    const long long rem = p - (long long)i * P.sz;
+    const int j = (int)(rem / P.sx), k = (int)(rem - (long long)j * P.sx);
+#else
+    const int i = (int)(p / P.sz), j = (int)(p - (long long)i * P.sz), k = 0;
+#endif
+    qs[s] = (unsigned)lo;
+    ws[s] = box_weight(rbox[lo], rw, i, j, k);
+}
+
+// Per reverse step: each touched point adds its receivers' terms in
+// receiver order (receivers whose residual is zero add nothing, as in
+// k_adj_receivers).
+__global__ void __launch_bounds__(256)
+k_adj_rec_gather(const unsigned *__restrict__ pts, const unsigned *__restrict__ start,
+                 const unsigned *__restrict__ cnt, int nu, const unsigned *__restrict__ qs,
+                 const real_t *__restrict__ ws, const real_t *__restrict__ res,
+                 real_t *__restrict__ ubar)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu)
+        return;
+    const unsigned p = pts[u];
+    real_t acc = ubar[p];
+    unsigned s = start[u];
+    const unsigned s1 = s + cnt[u];
+    // four entries' loads in flight, then their terms in order
+    for (; s + 4 <= s1; s += 4) {
+        real_t w[4], rv[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            w[t] = ws[s + t];
+            rv[t] = res[qs[s + t]];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            if (rv[t] != (real_t)0)
+                acc += w[t] * rv[t];
+    }
+    for (; s < s1; s++) {
+        const real_t rv = res[qs[s]];
+        if (rv != (real_t)0)
+            acc += ws[s] * rv;
+    }
+    ubar[p] = acc;
+}
+
 // Transpose of one boundary phase (k_bc_phase): per face line, the after
 // face's Neumann copies, then its Dirichlet zero, then the same for the
 // before face.

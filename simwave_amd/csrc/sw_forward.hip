@@ -18,6 +18,8 @@
 #include "sw_step3d.hpp"
 #include "sw_adjoint.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -1789,6 +1791,94 @@ void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *
 // `grad` (float64, one value per extended grid point) the velocity
 // gradient and writes into `gw` ([T][num_sources], device, or null) the
 // wavelet gradient.
+// The receiver-adjoint spread as a gather (k_adj_rec_gather): the touched
+// points, where each one's entries start, how many, and the sorted entries'
+// receivers and weights.  nu == 0: the atomic spread (k_adj_receivers) is
+// used instead (no receivers, more than 2^31 points or entries, or
+// SIMWAVE_HIP_REC_ATOMIC set).
+struct RecGather {
+    const unsigned *pts = nullptr, *start = nullptr, *cnt = nullptr, *qs = nullptr;
+    const real_t *ws = nullptr;
+    int nu = 0;
+};
+
+RecGather build_rec_gather(const Args &a, const Prepared &pr, DeviceArena &arena,
+                           hipStream_t st)
+{
+    RecGather g;
+    if (!a.nrec || env_flag("SIMWAVE_HIP_REC_ATOMIC"))
+        return g;
+    const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : 1};
+    const std::vector<Box> rb = make_boxes("receiver", a.nrec, a.rint, a.rint_size, a.roff,
+                                           a.rvals_size, dims);
+    std::vector<unsigned> off(a.nrec + 1);
+    unsigned long long tot = 0;
+    for (size_t q = 0; q < a.nrec; q++) {
+        off[q] = (unsigned)tot;
+        unsigned long long cnt = 1;
+        for (int d = 0; d < SW_DIM; d++)
+            cnt *= (unsigned long long)(rb[q].hi[d] - rb[q].lo[d] + 1);
+        tot += cnt;
+        if (tot >= (1ull << 31))
+            return g;
+    }
+    if (pr.npts >= (1ll << 31) || a.nrec >= (1u << 31))
+        return g;
+    off[a.nrec] = (unsigned)tot;
+    const unsigned n = (unsigned)tot;
+    const int nrec = (int)a.nrec;
+    unsigned *d_off = arena.alloc<unsigned>(a.nrec + 1);
+    hip_check(hipMemcpyAsync(d_off, off.data(), (a.nrec + 1) * sizeof(unsigned),
+                             hipMemcpyHostToDevice, st), "copy receiver entry offsets");
+    unsigned *key = arena.alloc<unsigned>(n), *idx = arena.alloc<unsigned>(n);
+    unsigned *key_s = arena.alloc<unsigned>(n), *idx_s = arena.alloc<unsigned>(n);
+    hipLaunchKernelGGL(sw::k_adj_rec_entries, dim3((unsigned)nrec), dim3(256), 0, st, pr.P,
+                       pr.rbox, nrec, d_off, key, idx);
+    hip_check(hipGetLastError(), "k_adj_rec_entries launch");
+    int end_bit = 1;
+    while (end_bit < 32 && (1ll << end_bit) < pr.npts)
+        end_bit++;
+    // a stable sort: entries of one point keep their receiver order
+    size_t tb = 0;
+    hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, idx_s, (int)n,
+                                                 0, end_bit, st), "radix sort size");
+    void *tmp = arena.alloc<unsigned char>(std::max<size_t>(tb, 1));
+    hip_check(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key_s, idx, idx_s, (int)n, 0,
+                                                 end_bit, st), "radix sort");
+    unsigned *pts = arena.alloc<unsigned>(n), *cnt = arena.alloc<unsigned>(n);
+    int *d_nu = arena.alloc<int>(1);
+    size_t tb2 = 0;
+    hip_check(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, key_s, pts, cnt, d_nu, (int)n,
+                                                    st), "run-length size");
+    void *tmp2 = arena.alloc<unsigned char>(std::max<size_t>(tb2, 1));
+    hip_check(hipcub::DeviceRunLengthEncode::Encode(tmp2, tb2, key_s, pts, cnt, d_nu, (int)n,
+                                                    st), "run-length encode");
+    int nu = 0;
+    hip_check(hipMemcpyAsync(&nu, d_nu, sizeof(int), hipMemcpyDeviceToHost, st),
+              "copy point count");
+    hip_check(hipStreamSynchronize(st), "receiver entries");
+    unsigned *start = arena.alloc<unsigned>((size_t)std::max(nu, 1));
+    size_t tb3 = 0;
+    hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, cnt, start, nu, st), "scan size");
+    void *tmp3 = arena.alloc<unsigned char>(std::max<size_t>(tb3, 1));
+    hip_check(hipcub::DeviceScan::ExclusiveSum(tmp3, tb3, cnt, start, nu, st), "scan");
+    unsigned *qs = arena.alloc<unsigned>(n);
+    real_t *ws = arena.alloc<real_t>(n);
+    hipLaunchKernelGGL(sw::k_adj_rec_permute, dim3((n + 255) / 256), dim3(256), 0, st, pr.P,
+                       pr.rbox, pr.rw, nrec, d_off, key_s, idx_s, n, qs, ws);
+    hip_check(hipGetLastError(), "k_adj_rec_permute launch");
+    for (void *p : {(void *)key, (void *)idx, (void *)key_s, (void *)idx_s, tmp, tmp2, tmp3,
+                    (void *)d_off, (void *)d_nu})
+        arena.release(p);
+    g.pts = pts;
+    g.start = start;
+    g.cnt = cnt;
+    g.qs = qs;
+    g.ws = ws;
+    g.nu = nu;
+    return g;
+}
+
 double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *gw, size_t K,
                    hipStream_t st, unsigned flags)
 {
@@ -1826,6 +1916,7 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     Prepared pr = prepare(a, true, 0u, arena, st);
     if (pr.halo_sources)
         fail("the adjoint needs every source box inside the interior");
+    const RecGather rg = build_rec_gather(a, pr, arena, st);
     const Params &P = pr.P;
     const size_t npts = (size_t)pr.npts;
     const size_t T = a.n1;
@@ -1911,7 +2002,11 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
             real_t *ub = ubar + (k % 3) * npts;
             real_t *ub1 = ubar + ((k - 1) % 3) * npts;
             real_t *ub2 = ubar + ((k - 2) % 3) * npts;
-            if (k <= T && a.nrec)
+            if (k <= T && a.nrec && rg.nu)
+                hipLaunchKernelGGL(sw::k_adj_rec_gather, dim3((unsigned)((rg.nu + 255) / 256)),
+                                   dim3(256), 0, st, rg.pts, rg.start, rg.cnt, rg.nu, rg.qs,
+                                   rg.ws, rec_adj + (k - 1) * a.nrec, ub);
+            else if (k <= T && a.nrec)
                 hipLaunchKernelGGL(sw::k_adj_receivers, dim3((unsigned)a.nrec), dim3(256), 0,
                                    st, P, pr.rbox, pr.rw, (int)a.nrec,
                                    rec_adj + (k - 1) * a.nrec, ub);
