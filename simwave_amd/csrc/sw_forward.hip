@@ -584,16 +584,16 @@ constexpr int default_tile_zlen(int R)
 // ~50 M interior points (1.3x at 22 M, even at 38 M, tiled 7% ahead at 60 M
 // and 9% at 100 M); radius 5+ never (its register queue is 2R+1 deep).
 // Float64: the tiled kernel (z-marching 26% slower at 22 M, radius 4).
-// Variable density (float32): at every radius and size measured, 1.2-1.8x
-// faster than the LDS-tiled k_step3d (176^3-448^3, orders 2-16); float64
-// keeps k_step3d, whose doubles the z-marching register queues would make
-// too large.
+// Variable density: float32 at every radius and size measured, 1.2-1.8x
+// faster than the LDS-tiled k_step3d (176^3-448^3, orders 2-16); float64 at
+// radius <= 4 (272^3 so8: 527 vs 690 us; 268^3 so4: 346 vs 413 us), above
+// which its double-precision register queues would spill.
 bool prefer_zmarch(const Prepared &pr)
 {
+    if (SW_VD)
+        return sizeof(real_t) == 4 || pr.r <= 4;
     if (sizeof(real_t) != 4)
         return false;
-    if (SW_VD)
-        return true;
     const Params &P = pr.P;
     const int R = pr.r;
     const long long interior = (long long)(P.nz - 2 * R) * (P.nx - 2 * R) * (P.ny - 2 * R);
@@ -796,11 +796,11 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
 #else
 constexpr int kTB = 256;
 
-// Whether 2D steps run k_step2d_z (see launch_fused).
-bool zmarch_2d()
+// Whether 2D steps of radius R run k_step2d_z (see launch_fused).
+bool zmarch_2d(int R)
 {
     return env_flag("SIMWAVE_HIP_ZMARCH") ||
-           (sizeof(real_t) == 4 && !env_flag("SIMWAVE_HIP_UNTILED"));
+           ((sizeof(real_t) == 4 || R <= 4) && !env_flag("SIMWAVE_HIP_UNTILED"));
 }
 
 template <int R>
@@ -819,11 +819,13 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
         zlen = (Iz + nzc - 1) / nzc;
     }
     const int gz = (Iz + zlen - 1) / zlen;
-    // float32: the z-marching kernel (0-27% faster than the LDS row kernel
-    // on 2D grids of 304^2 to 1136 x 3136, orders 2-20, CD and VD: 41.7 vs
-    // 57.0 us per step at 1136 x 3136 so16 VD); float64 keeps k_step2d
-    // (SIMWAVE_HIP_UNTILED forces it, SIMWAVE_HIP_ZMARCH the z-marching one)
-    if (zmarch_2d()) {
+    // the z-marching kernel for float32 (0-27% faster than the LDS row
+    // kernel on 2D grids of 304^2 to 1136 x 3136, orders 2-20, CD and VD:
+    // 41.7 vs 57.0 us per step at 1136 x 3136 so16 VD) and for float64 up
+    // to radius 4 (517^2 so4: 8.2 vs 8.7 us); k_step2d for float64 above
+    // (1136 x 3136 so16 VD: 66.5 vs 71.3 us).  SIMWAVE_HIP_UNTILED forces
+    // k_step2d, SIMWAVE_HIP_ZMARCH the z-marching one.
+    if (zmarch_2d(R)) {
         static_assert(sw::kZ2Block == kTB, "2D launch shapes differ");
         hipLaunchKernelGGL(sw::k_step2d_z<R>, dim3(gx, gz), dim3(sw::kZ2Block), 0, st,
                            P, uc, up, un, n, zlen);
@@ -1525,7 +1527,7 @@ double run_host(const Args &a)
 #if SW_DIM == 3
         g_timings[7] = pr.tiled ? 1.0 : pr.zmarch ? 2.0 : 0.0;
 #else
-        g_timings[7] = pr.fused && zmarch_2d() ? 2.0 : 0.0;
+        g_timings[7] = pr.fused && zmarch_2d(pr.r) ? 2.0 : 0.0;
 #endif
     }
     const double total = now_s() - t0;
@@ -1707,7 +1709,7 @@ size_t run_plan(Plan &p, size_t count, hipStream_t st)
 #if SW_DIM == 3
     g_timings[7] = p.pr.tiled ? 1.0 : p.pr.zmarch ? 2.0 : 0.0;
 #else
-    g_timings[7] = p.pr.fused && zmarch_2d() ? 2.0 : 0.0;
+    g_timings[7] = p.pr.fused && zmarch_2d(p.pr.r) ? 2.0 : 0.0;
 #endif
     return p.slots.size() - p.done;
 }

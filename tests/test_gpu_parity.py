@@ -25,6 +25,7 @@ CASES = {
     "2d_vd_so16": dict(dim=2, space_order=16, density=True, n=40),
     "2d_cd_f64_so8": dict(dim=2, space_order=8, dtype=F64),
     "2d_vd_f64_so8": dict(dim=2, space_order=8, density=True, dtype=F64),
+    "2d_vd_f64_so16": dict(dim=2, space_order=16, density=True, dtype=F64, n=40),
     # boundary codes on every face
     "2d_bc_none": dict(dim=2, bc=("none",) * 4, damping=0.0, tf=0.3),
     "2d_bc_dirichlet": dict(dim=2, bc=("null_dirichlet",) * 4, damping=0.0, tf=0.3),
@@ -61,6 +62,8 @@ CASES = {
     "3d_cd_f64_so8": dict(dim=3, n=26, space_order=8, dtype=F64, tf=0.08),
     "3d_vd_f64_so4": dict(dim=3, n=26, space_order=4, density=True, dtype=F64,
                           tf=0.08),
+    "3d_vd_f64_so12": dict(dim=3, n=30, space_order=12, density=True, dtype=F64,
+                           tf=0.05),
     "3d_bc_neumann": dict(dim=3, n=26, bc=("null_neumann",) * 6, damping=0.0,
                           tf=0.15),
     "3d_bc_dirichlet": dict(dim=3, n=26, bc=("null_dirichlet",) * 6, damping=0.0,
@@ -279,14 +282,16 @@ def _step_kernel_of_last_run(dim, dens, prec):
 # (case, tile forced, expected 3D step kernel): constant-density float32
 # grids of radius <= 3, and small ones of radius 4, take the z-marching
 # kernel; radius > 4 and float64 the tiled one; a forced tile configuration
-# always the tiled one.  Variable density: z-marching for float32,
-# k_step3d for float64.  2D: z-marching for float32, k_step2d for float64.
+# always the tiled one.  Variable density and 2D: z-marching for float32
+# and for float64 up to radius 4, the LDS kernels (k_step3d, k_step2d)
+# above.
 KERNEL_CHOICE = [("3d_cd_so8", None, 2), ("3d_cd_so2", None, 2), ("3d_wide_so6", None, 2),
                  ("3d_cd_so20", None, 1),
                  ("3d_cd_f64_so8", None, 1), ("3d_cd_so8", "10", 1),
                  ("3d_vd_so8", None, 2), ("3d_vd_so16_wide", None, 2),
-                 ("3d_vd_f64_so4", None, 0),
-                 ("2d_cd_so8", None, 2), ("2d_vd_so16", None, 2), ("2d_cd_f64_so8", None, 0)]
+                 ("3d_vd_f64_so4", None, 2), ("3d_vd_f64_so12", None, 0),
+                 ("2d_cd_so8", None, 2), ("2d_vd_so16", None, 2), ("2d_cd_f64_so8", None, 2),
+                 ("2d_vd_f64_so16", None, 0)]
 
 
 @pytest.mark.parametrize("name,tile,expect", KERNEL_CHOICE)
@@ -304,10 +309,12 @@ def test_default_step_kernel_choice(gpu, name, tile, expect):
 
 
 # The LDS kernels k_step3d and k_step2d (SIMWAVE_HIP_UNTILED): the float64
-# variable-density 3D and float64 2D defaults, forced here for float32 too.
+# defaults above radius 4 (3D variable density, 2D), forced here for the
+# other cases too.
 UNTILED = ["3d_vd_so4", "3d_vd_so8", "3d_vd_bc", "3d_vd_so16_wide", "3d_cd_so8",
            "3d_bc_mixed", "3d_src_halo", "2d_cd_so2", "2d_cd_so8", "2d_cd_so20",
-           "2d_vd_so4", "2d_vd_so16", "2d_bc_mixed", "2d_src_halo", "2d_stride3"]
+           "2d_vd_so4", "2d_vd_so16", "2d_bc_mixed", "2d_src_halo", "2d_stride3",
+           "2d_cd_f64_so8", "3d_vd_f64_so4"]
 
 
 @pytest.mark.parametrize("name", UNTILED)
