@@ -107,7 +107,7 @@ constexpr size_t kGraphSteps = 48;   // a multiple of the 3-slot cycle
 constexpr unsigned kAdjImage = 1u;
 constexpr int kMaxFusedSources = 64;
 // cooperative receiver kernel: receivers per block and points per round
-// (LDS = kRecChunk * kRecPerBlock values)
+// (LDS = kRecChunk * (kRecPerBlock + 1) values)
 constexpr int kRecPerBlock = sizeof(real_t) == 4 ? 16 : 8;
 constexpr int kRecChunk = 768;
 // grids from this many points on run the receiver gather on a second stream

@@ -1045,7 +1045,10 @@ k_receivers(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbo
 // Receivers, cooperative form: a block serves RPB receivers.  All 256
 // threads gather box values and multiply them by their weights into LDS (CH
 // points per receiver per round), then lane q of the first wave adds its
-// receiver's products in box order.  Every product and every partial sum is
+// receiver's products in box order.  The 256/RPB threads of one receiver
+// are consecutive, so a wavefront's loads walk along box rows (contiguous
+// y) of a few receivers; the LDS rows are padded by one word against bank
+// conflicts.  Every product and every partial sum is
 // rounded exactly as in the reference's sequential loop (wave.c:533-565); only
 // the loads and multiplies run in parallel.
 template <int RPB, int CH>
@@ -1055,7 +1058,7 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
 {
     if (P.nbase)
         out += (long long)(*P.nbase) * nrec;
-    __shared__ real_t prod[CH][RPB];
+    __shared__ real_t prod[CH][RPB + 1];
     __shared__ Box boxes[RPB];
     __shared__ int counts[RPB];
     __shared__ int maxcount;
@@ -1082,7 +1085,7 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
     __syncthreads();
     const int total_max = maxcount;
     constexpr int STEP = 256 / RPB;       // points advanced per thread per pass
-    const int l = tid % RPB;              // receiver served by this thread
+    const int l = tid / STEP;             // receiver served by this thread
     const Box b = boxes[l];
     const int cnt = counts[l];
     const int cz = b.hi[0] - b.lo[0] + 1, cx = b.hi[1] - b.lo[1] + 1;
@@ -1099,7 +1102,7 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
     real_t acc = 0;
     for (int base = 0; base < total_max; base += CH) {
         // this thread's first point of the round, decoded once, then walked
-        int t = base + tid / RPB;
+        int t = base + tid % STEP;
         int ii = 0, jj = 0, kk = 0;
         if (t < cnt) {
             ii = t / (cx * cy);
@@ -1107,7 +1110,7 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
             jj = rem / cy;
             kk = rem - jj * cy;
         }
-        for (int c = tid / RPB; c < CH && t < cnt; c += STEP, t += STEP) {
+        for (int c = tid % STEP; c < CH && t < cnt; c += STEP, t += STEP) {
 #if SW_DIM == 3
             const real_t kws = wz[ii] * wx[jj] * wy[kk];
             const long long p = (long long)(b.lo[0] + ii) * P.sz +
@@ -1128,7 +1131,8 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
         }
         __syncthreads();
         if (tid < RPB) {
-            const int nround = min(CH, cnt - base);
+            // lane tid sums receiver tid (its loads above served receiver l)
+            const int nround = min(CH, counts[tid] - base);
             for (int c = 0; c < nround; c++)
                 acc += prod[c][tid];
         }
