@@ -129,6 +129,23 @@ k_adj_receivers(Params P, const Box *__restrict__ rbox, const real_t *__restrict
     }
 }
 
+// grad += gacc and gacc = 0 where gacc holds anything: the float64
+// gradient takes the working-precision sums of the last few reverse steps
+// (the update kernels add each step's term to gacc, which halves their
+// gradient traffic in float32).
+__global__ void __launch_bounds__(256)
+k_grad_flush(real_t *__restrict__ gacc, double *__restrict__ grad, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const real_t g = gacc[i];
+        if (g != (real_t)0) {
+            grad[i] += (double)g;
+            gacc[i] = (real_t)0;
+        }
+    }
+}
+
 // The same spread as a gather, with no atomics and a fixed order.  Built
 // once per gradient: every (point, receiver) pair of the receiver boxes as
 // an entry, sorted by point (a stable sort of entries made in receiver
@@ -325,7 +342,7 @@ __global__ void __launch_bounds__(64 * kAdjLines)
 k_adj_update(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
              const real_t *__restrict__ W, real_t *__restrict__ ub1, real_t *__restrict__ ub2,
              const real_t *__restrict__ u0, const real_t *__restrict__ u1,
-             const real_t *__restrict__ u2, double *__restrict__ grad, int image)
+             const real_t *__restrict__ u2, real_t *__restrict__ gacc, int image)
 {
     int i, j, k;
     if (thread_point(P, i, j, k)) {
@@ -346,12 +363,12 @@ k_adj_update(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub,
         if (w != (real_t)0) {
             const double x0 = (double)u0[p], x1 = (double)u1[p], x2 = (double)u2[p];
             if (image) {
-                grad[p] += (double)w * x0;
+                gacc[p] += (real_t)((double)w * x0);
             } else {
                 const double rv = 1.0 / (double)vel[p];
                 const double da = -4.0 * e * rden * rden * rv;
                 const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
-                grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+                gacc[p] += (real_t)((double)w * (da * (x1 - x2) + 2.0 * rden * rv * step));
             }
         }
     }
@@ -375,7 +392,7 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
                const real_t *__restrict__ W, real_t *__restrict__ ub1,
                real_t *__restrict__ ub2, const real_t *__restrict__ u0,
                const real_t *__restrict__ u1, const real_t *__restrict__ u2,
-               double *__restrict__ grad, int image)
+               real_t *__restrict__ gacc, int image)
 {
     const int j = blockIdx.y * kAdjLines + threadIdx.y;
     const int k = blockIdx.x * 64 + threadIdx.x;
@@ -460,12 +477,12 @@ k_adj_update_z(Params P, const real_t *__restrict__ vel, real_t *__restrict__ ub
             if (w != (real_t)0) {
                 const double x0 = (double)cur.x0, x1 = (double)cur.x1, x2 = (double)cur.x2;
                 if (image) {
-                    grad[p] += (double)w * x0;
+                    gacc[p] += (real_t)((double)w * x0);
                 } else {
                     const double rv = 1.0 / (double)cur.v;
                     const double da = -4.0 * e * rden * rden * rv;
                     const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
-                    grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+                    gacc[p] += (real_t)((double)w * (da * (x1 - x2) + 2.0 * rden * rv * step));
                 }
             }
         }
@@ -487,7 +504,7 @@ k_adj_update_z2d(Params P, const real_t *__restrict__ vel, real_t *__restrict__ 
                  const real_t *__restrict__ W, real_t *__restrict__ ub1,
                  real_t *__restrict__ ub2, const real_t *__restrict__ u0,
                  const real_t *__restrict__ u1, const real_t *__restrict__ u2,
-                 double *__restrict__ grad, int image)
+                 real_t *__restrict__ gacc, int image)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= P.nx)
@@ -560,12 +577,12 @@ k_adj_update_z2d(Params P, const real_t *__restrict__ vel, real_t *__restrict__ 
             if (w != (real_t)0) {
                 const double x0 = (double)cur.x0, x1 = (double)cur.x1, x2 = (double)cur.x2;
                 if (image) {
-                    grad[p] += (double)w * x0;
+                    gacc[p] += (real_t)((double)w * x0);
                 } else {
                     const double rv = 1.0 / (double)cur.v;
                     const double da = -4.0 * e * rden * rden * rv;
                     const double step = x0 - a * x1 + b * x2;     // c (L u1 + s)
-                    grad[p] += (double)w * (da * (x1 - x2) + 2.0 * rden * rv * step);
+                    gacc[p] += (real_t)((double)w * (da * (x1 - x2) + 2.0 * rden * rv * step));
                 }
             }
         }

@@ -105,6 +105,9 @@ constexpr size_t kGraphSteps = 48;   // a multiple of the 3-slot cycle
 // adjoint entry: accumulate the zero-lag cross-correlation image of the
 // forward and adjoint wavefields instead of the velocity gradient
 constexpr unsigned kAdjImage = 1u;
+// reverse steps whose gradient terms are summed in the working precision
+// before they are added to the float64 gradient
+constexpr size_t kGradFlush = 16;
 constexpr int kMaxFusedSources = 64;
 // cooperative receiver kernel: receivers per block and points per round
 // (LDS = kRecChunk * (kRecPerBlock + 1) values)
@@ -1738,25 +1741,25 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
 template <int R>
 void adj_update_z(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *ub,
                   const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
-                  const real_t *u1, const real_t *u2, double *grad, int image)
+                  const real_t *u1, const real_t *u2, real_t *gacc, int image)
 {
     const dim3 grid((unsigned)((P.ny + 63) / 64),
                     (unsigned)((P.nx + sw::kAdjLines - 1) / sw::kAdjLines),
                     (unsigned)((P.nz + sw::kAdjZChunk - 1) / sw::kAdjZChunk));
     hipLaunchKernelGGL(sw::k_adj_update_z<R>, grid, sw::adj_block(), 0, st, P, vel, ub, W,
-                       ub1, ub2, u0, u1, u2, grad, image);
+                       ub1, ub2, u0, u1, u2, gacc, image);
 }
 #endif
 
 void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *ub,
                 const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
-                const real_t *u1, const real_t *u2, double *grad, int image)
+                const real_t *u1, const real_t *u2, real_t *gacc, int image)
 {
 #if SW_DIM == 3
     switch (P.r) {
 #define SW_ADJ_CASE(R)                                                              \
     case R:                                                                         \
-        adj_update_z<R>(P, st, vel, ub, W, ub1, ub2, u0, u1, u2, grad, image);      \
+        adj_update_z<R>(P, st, vel, ub, W, ub1, ub2, u0, u1, u2, gacc, image);      \
         return;
     SW_ADJ_CASE(1) SW_ADJ_CASE(2) SW_ADJ_CASE(3) SW_ADJ_CASE(4) SW_ADJ_CASE(5)
     SW_ADJ_CASE(6) SW_ADJ_CASE(7) SW_ADJ_CASE(8) SW_ADJ_CASE(9) SW_ADJ_CASE(10)
@@ -1771,7 +1774,7 @@ void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *
         hipLaunchKernelGGL(sw::k_adj_update_z2d<R>,                                \
                            dim3((unsigned)((P.nx + 255) / 256),                     \
                                 (unsigned)((P.nz + sw::kAdjZChunk - 1) / sw::kAdjZChunk)), \
-                           dim3(256), 0, st, P, vel, ub, W, ub1, ub2, u0, u1, u2, grad, image); \
+                           dim3(256), 0, st, P, vel, ub, W, ub1, ub2, u0, u1, u2, gacc, image); \
         return;
     SW_ADJ_CASE(1) SW_ADJ_CASE(2) SW_ADJ_CASE(3) SW_ADJ_CASE(4) SW_ADJ_CASE(5)
     SW_ADJ_CASE(6) SW_ADJ_CASE(7) SW_ADJ_CASE(8) SW_ADJ_CASE(9) SW_ADJ_CASE(10)
@@ -1781,7 +1784,7 @@ void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *
     }
 #endif
     hipLaunchKernelGGL(sw::k_adj_update, sw::adj_grid(P), sw::adj_block(), 0, st, P, vel, ub, W,
-                       ub1, ub2, u0, u1, u2, grad, image);
+                       ub1, ub2, u0, u1, u2, gacc, image);
 }
 
 // Shot record and its adjoint: runs the forward of a constant-density shot
@@ -1936,8 +1939,18 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     real_t *seg = arena.alloc<real_t>((K + 2) * npts);
     real_t *ubar = arena.alloc<real_t>(3 * npts);
     real_t *W = arena.alloc<real_t>(npts);
+    // the reverse steps add their gradient terms here, in the working
+    // precision; every kGradFlush steps (counted over the whole sweep, so
+    // the sums do not depend on K) and at the end they go into grad
+    real_t *gacc = arena.alloc<real_t>(npts);
+    size_t reverse_steps = 0;
+    auto flush_grad = [&]() {
+        hipLaunchKernelGGL(sw::k_grad_flush, dim3(nb), dim3(256), 0, st, gacc, grad,
+                           (long long)npts);
+    };
     zero(seg, (K + 2) * npts);
     zero(ubar, 3 * npts);
+    zero(gacc, npts);
     hipLaunchKernelGGL(sw::k_fill_f64, dim3(nb), dim3(256), 0, st, grad, (long long)npts);
     if (gw && a.nsrc)
         hipLaunchKernelGGL(sw::k_fill, dim3(1), dim3(256), 0, st, gw,
@@ -2033,12 +2046,15 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
                 hipLaunchKernelGGL(sw::k_adj_sources, dim3((unsigned)a.nsrc), dim3(256), 0, st,
                                    P, ub, gw + (n - 1) * a.nsrc);
             adj_update(P, st, a.vel, ub, W, ub1, ub2, seg + (n + 1 - m) * npts,
-                       seg + (n - m) * npts, seg + (n - 1 - m) * npts, grad, image ? 1 : 0);
+                       seg + (n - m) * npts, seg + (n - 1 - m) * npts, gacc, image ? 1 : 0);
+            if (++reverse_steps % kGradFlush == 0)
+                flush_grad();
             hip_check(hipGetLastError(), "adjoint step");
             if (n == n_lo)
                 break;
         }
     }
+    flush_grad();
     hip_check(hipStreamSynchronize(st), "adjoint");
     g_timings[1] = t_fwd - t0;
     g_timings[2] = now_s() - t_fwd;
