@@ -311,19 +311,17 @@ __global__ void k_adj_bc_phase(Params P, real_t *ub, int axis)
         ub[lo] = 0;
 }
 
-// W = c * ubar on the interior, 0 elsewhere.
+// W = c * ubar on the interior, 0 elsewhere, with c = C = q / den the
+// per-point coefficient the forward step multiplies the Laplacian by
+// (k_scaled_q; wave.c:177-185).
 __global__ void __launch_bounds__(64 * kAdjLines)
-k_adj_cw(Params P, const real_t *__restrict__ ub, real_t *__restrict__ W)
+k_adj_cw(Params P, const real_t *__restrict__ C, const real_t *__restrict__ ub,
+         real_t *__restrict__ W)
 {
     int i, j, k;
     if (thread_point(P, i, j, k)) {
         const long long p = (long long)i * P.sz + (long long)j * P.sx + k;
-        real_t w = 0;
-        if (interior(P, i, j, k)) {
-            const double den = 1.0 + (double)P.e[p];
-            w = (real_t)((double)P.q[p] / den * (double)ub[p]);
-        }
-        W[p] = w;
+        W[p] = interior(P, i, j, k) ? C[p] * ub[p] : (real_t)0;
     }
 }
 

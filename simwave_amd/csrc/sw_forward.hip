@@ -1944,6 +1944,19 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     // the sums do not depend on K) and at the end they go into grad
     real_t *gacc = arena.alloc<real_t>(npts);
     size_t reverse_steps = 0;
+    // the forward's coefficient C = q / den per point (the tiled kernel's
+    // copy where it was built)
+#if SW_DIM == 3
+    const real_t *Cq = pr.qc;
+#else
+    const real_t *Cq = nullptr;
+#endif
+    if (!Cq) {
+        real_t *c = arena.alloc<real_t>(npts);
+        hipLaunchKernelGGL(sw::k_scaled_q, dim3(nb), dim3(256), 0, st, P.q, P.e, c,
+                           (long long)npts);
+        Cq = c;
+    }
     auto flush_grad = [&]() {
         hipLaunchKernelGGL(sw::k_grad_flush, dim3(nb), dim3(256), 0, st, gacc, grad,
                            (long long)npts);
@@ -2041,7 +2054,8 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
                 hipLaunchKernelGGL(sw::k_adj_bc_phase,
                                    dim3((unsigned)((lines[ph] + 255) / 256)), dim3(256), 0, st,
                                    P, ub, axes[ph]);
-            hipLaunchKernelGGL(sw::k_adj_cw, sw::adj_grid(P), sw::adj_block(), 0, st, P, ub, W);
+            hipLaunchKernelGGL(sw::k_adj_cw, sw::adj_grid(P), sw::adj_block(), 0, st, P, Cq, ub,
+                               W);
             if (a.nsrc && gw)
                 hipLaunchKernelGGL(sw::k_adj_sources, dim3((unsigned)a.nsrc), dim3(256), 0, st,
                                    P, ub, gw + (n - 1) * a.nsrc);
