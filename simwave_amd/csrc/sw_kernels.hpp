@@ -474,6 +474,9 @@ k_step3d(Params P, const real_t *__restrict__ uc, const real_t *up,
 // few tiles to keep that kernel's 2 per SIMD busy.  Variable density: the
 // fused path requires nx == ny (wave.c's x-axis first derivatives step by
 // nx), so the x neighbours at m * sx are the reference's.
+// rows (of 64 threads) per block; 4 and 16 rows measured: 16 slower
+// everywhere, 4 faster on shallow grids (228 x 324 x 324 so8: 9%) and
+// slower on deep so8 cubes (176^3-352^3: 1-2.5%)
 constexpr int kZLines = 8;
 
 template <int R>
