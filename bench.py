@@ -535,8 +535,11 @@ def time_cpu(kwargs, variant, target_s=15.0):
             "seconds": round(secs, 3)}
 
 
-def read_traffic(kernel, workload):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+def read_traffic(kernel, workload, dtype):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary (float32
+    runs; no summary covers float64)."""
+    if dtype != np.float32:
+        return None
     path = os.path.join(HERE, "profiles", "%s_%s_traffic.json" % (kernel, workload))
     if not os.path.exists(path):
         return None
@@ -664,7 +667,7 @@ def main():
             kname = "k_step%dd" % ndim
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic(kname, args.workload),
+                "traffic": read_traffic(kname, args.workload, dtype),
                 "kernel": kname, "avg_launch_us": round(avg * 1e6, 2),
                 "launches": int(tm[5]), "algorithmic_bytes_per_launch": pts * bpp}
         log("step kernel: %d launches, avg %.1f us, %.1f GB/s algorithmic"
