@@ -16,7 +16,9 @@
  *   - simwave_hip_plan_{create,run,destroy}_<variant>: the device entry
  *     split into chunks for serving loops;
  *   - simwave_hip_fixed_div_<variant>, simwave_hip_last_error,
- *     simwave_hip_last_timings, simwave_hip_variant.
+ *     simwave_hip_last_timings, simwave_hip_variant, simwave_hip_trim_pool;
+ *   - through libsimwave_hip_common.so, which every variant library links:
+ *     simwave_hip_overlap_{acquire,release,holders}.
  *
  * Array layout (reference contract, middleware.py:160-285): C-contiguous,
  * last axis fastest; u is [num_snapshots][nz][nx][ny] (2D: no ny);
@@ -63,8 +65,8 @@ double simwave_hip_forward_2d_cd_f32(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference constant_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -125,7 +127,9 @@ int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_cd_f32(
     float *u, float *velocity, float *damp,
     float *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -172,8 +176,8 @@ double simwave_hip_forward_2d_cd_f64(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference constant_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -234,7 +238,9 @@ int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_cd_f64(
     double *u, double *velocity, double *damp,
     double *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -281,8 +287,8 @@ double simwave_hip_forward_2d_vd_f32(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference variable_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -343,7 +349,9 @@ int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
     float *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -390,8 +398,8 @@ double simwave_hip_forward_2d_vd_f64(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference variable_density/2d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_2d_vd_f64(
     double *u, double *velocity, double *density, double *damp,
@@ -452,7 +460,9 @@ int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_2d_vd_f64(
     double *u, double *velocity, double *density, double *damp,
     double *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -499,8 +509,8 @@ double simwave_hip_forward_3d_cd_f32(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference constant_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_cd_f32(
     float *u, float *velocity, float *damp,
@@ -561,7 +571,9 @@ int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_cd_f32(
     float *u, float *velocity, float *damp,
     float *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -608,8 +620,8 @@ double simwave_hip_forward_3d_cd_f64(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference constant_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_cd_f64(
     double *u, double *velocity, double *damp,
@@ -670,7 +682,9 @@ int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_cd_f64(
     double *u, double *velocity, double *damp,
     double *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -717,8 +731,8 @@ double simwave_hip_forward_3d_vd_f32(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference variable_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
@@ -779,7 +793,9 @@ int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_vd_f32(
     float *u, float *velocity, float *density, float *damp,
     float *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -826,8 +842,8 @@ double simwave_hip_forward_3d_vd_f64(
  * stream) and returns after the stream has drained.  The models are only
  * read.  flags: 1 time the step kernel with HIP events, 2 force the
  * generic kernels, 4 run the receiver gather of large grids on a second
- * stream of the library's (event-ordered; leave it off when many threads
- * drive many streams).  Same arguments as
+ * stream of the library's (event-ordered; one run or plan per process may
+ * hold it, see simwave_hip_overlap_acquire; an error while another does).  Same arguments as
  * reference variable_density/3d/wave.c:23-36, plus stream and flags. */
 double simwave_hip_forward_device_3d_vd_f64(
     double *u, double *velocity, double *density, double *damp,
@@ -888,7 +904,9 @@ int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
  * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
  * the forward's receiver gather on a second stream (large grids).
- * Also writes the shot record.  Returns wall seconds, or -1. */
+ * The shot starts from rest: u's three slots are zeroed first, whatever
+ * they hold.  Also writes the shot record and leaves u as after a plain
+ * forward.  Returns wall seconds, or -1. */
 double simwave_hip_gradient_3d_vd_f64(
     double *u, double *velocity, double *density, double *damp,
     double *wavelet, size_t wavelet_size, size_t wavelet_count,
@@ -922,6 +940,26 @@ const char *simwave_hip_last_error(void);
 int simwave_hip_last_timings(double *out, int n);
 /* The library's variant, e.g. "3d_cd_f32". */
 const char *simwave_hip_variant(void);
+/* No reference counterpart.  Hands the freed memory that the current
+ * device's default pool keeps for reuse (the library's stream-ordered
+ * allocations; up to 8 GiB is kept between runs) back to the device, after
+ * waiting for the device, so that a free-memory query sees it.  0, or -1. */
+int simwave_hip_trim_pool(void);
+
+/* ---- libsimwave_hip_common.so (linked by every variant library; one copy
+ * per process).  No reference counterpart.  The receiver-overlap lease:
+ * flag 4 joins a second stream of the library's with event waits in both
+ * directions, and such waits enqueued from several host threads at once
+ * can order into a cycle on the device's shared hardware queues, so one
+ * run or plan per process holds it.  The entry points take it themselves
+ * (flag 4 returns an error while another holds it; the host entry `forward`
+ * then runs its receivers on its one stream instead). */
+/* 1 if the caller now holds the lease, 0 if another holder has it. */
+int simwave_hip_overlap_acquire(void);
+/* Releases the lease. */
+void simwave_hip_overlap_release(void);
+/* Current holders (0 or 1). */
+int simwave_hip_overlap_holders(void);
 
 #ifdef __cplusplus
 }

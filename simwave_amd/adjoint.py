@@ -46,6 +46,11 @@ def checkpoint_interval(dev, checkpoint_every=None):
     T = dev.kwargs["end_timestep"]
     vel = dev.tensors["velocity_model"]
     need = (T + 2) * vel.numel() * vel.element_size()
+    # memory an earlier run freed into the library's pool counts as used in
+    # the query below until the pool hands it back
+    with torch.cuda.device(dev.device):
+        if dev._lib.simwave_hip_trim_pool() != 0:
+            raise RuntimeError("pool trim failed: {}".format(_last_error(dev._lib)))
     free = torch.cuda.mem_get_info(dev.device)[0]
     return T if need <= 0.8 * free else 20
 

@@ -48,6 +48,35 @@ def hipcc():
     return exe
 
 
+COMMON = "libsimwave_hip_common.so"
+
+
+def common_library_path():
+    return os.path.join(LIB_DIR, COMMON)
+
+
+def _compile_common(verbose=False):
+    """Process-wide state shared by the variant libraries (sw_common.cpp):
+    each variant links it, so one copy is mapped per process."""
+    out = common_library_path()
+    tmp = out + ".tmp%d" % os.getpid()
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("g++ not found; the common library cannot be built")
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+           "-Wl,-soname," + COMMON, os.path.join(CSRC, "sw_common.cpp"), "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        if os.path.exists(tmp):
+            os.remove(tmp)
+        raise RuntimeError("common library build failed:\n%s\n%s"
+                           % (" ".join(cmd), res.stderr))
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def _compile(variant, verbose=False):
     dim, dens, prec = variant
     out = library_path(dim, dens, prec)
@@ -58,7 +87,8 @@ def _compile(variant, verbose=False):
     ]
     if prec == "f64":
         cmd.append("-DSW_DOUBLE")
-    cmd += [os.path.join(CSRC, "sw_forward.hip"), "-o", tmp]
+    cmd += [os.path.join(CSRC, "sw_forward.hip"), "-o", tmp,
+            "-L" + LIB_DIR, "-l:" + COMMON, "-Wl,-rpath,$ORIGIN"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     errors = "\n".join(line for line in res.stderr.splitlines()
                        if "packed-fp32-ops" not in line)
@@ -74,8 +104,10 @@ def _compile(variant, verbose=False):
 
 
 def build(verbose=True, jobs=None):
-    """Compiles all eight variants (always; cached objects are not reused)."""
+    """Compiles the common library and all eight variants (always; cached
+    objects are not reused)."""
     os.makedirs(LIB_DIR, exist_ok=True)
+    _compile_common(verbose)
     jobs = jobs or min(len(VARIANTS), os.cpu_count() or 1, 8)
     with ThreadPoolExecutor(max_workers=jobs) as pool:
         outs = list(pool.map(lambda v: _compile(v, verbose), VARIANTS))

@@ -116,6 +116,35 @@ constexpr int kRecChunk = 768;
 // grids from this many points on run the receiver gather on a second stream
 constexpr long long kOverlapMinPoints = 1ll << 22;
 
+// The receiver-overlap lease (sw_common.cpp): one run or plan per process may
+// put its receiver gather on a second stream.  `required`: a caller that asked
+// for the overlap gets an error while another holds it; otherwise the run
+// quietly keeps its receivers on its one stream.
+extern "C" int simwave_hip_overlap_acquire(void);
+extern "C" void simwave_hip_overlap_release(void);
+
+struct OverlapLease {
+    bool held = false;
+    OverlapLease() = default;
+    OverlapLease(const OverlapLease &) = delete;
+    OverlapLease &operator=(const OverlapLease &) = delete;
+    bool acquire(bool required)
+    {
+        if (held)
+            return true;
+        held = simwave_hip_overlap_acquire() != 0;
+        if (!held && required)
+            fail("receiver overlap (flag 4) is already in use by another run or plan in "
+                 "this process; run without it, or after that run or plan has ended");
+        return held;
+    }
+    ~OverlapLease()
+    {
+        if (held)
+            simwave_hip_overlap_release();
+    }
+};
+
 // All arguments of one forward call, normalised across variants.
 struct Args {
     real_t *u;
@@ -223,16 +252,35 @@ std::vector<Box> make_boxes(const char *what, size_t count, const size_t *iv,
     return boxes;
 }
 
-// Stream-ordered device allocations released when the object dies.
+// Stream-ordered device allocations released when the object dies.  With a
+// base pointer it instead hands out consecutive 256-byte aligned pieces of
+// that caller-owned region (no allocation, nothing to free) and fails once
+// `cap` bytes are used.
 struct DeviceArena {
     hipStream_t stream;
     std::vector<void *> ptrs;
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
     explicit DeviceArena(hipStream_t s) : stream(s) {}
+    DeviceArena(hipStream_t s, void *region, size_t bytes)
+        : stream(s), base(static_cast<char *>(region)), cap(bytes)
+    {
+    }
     template <typename T>
     T *alloc(size_t count)
     {
         if (count == 0)
             return nullptr;
+        if (base) {
+            const size_t bytes = (count * sizeof(T) + 255) / 256 * 256;
+            if (bytes > cap - used)
+                fail("the run needs more than the %zu bytes of the library's static "
+                     "scratch (%zu used before a request of %zu bytes); use the "
+                     "allocating entry point for this grid", cap, used, count * sizeof(T));
+            void *p = base + used;
+            used += bytes;
+            return static_cast<T *>(p);
+        }
         void *p = nullptr;
         hip_check(hipMallocAsync(&p, count * sizeof(T), stream), "hipMallocAsync");
         ptrs.push_back(p);
@@ -254,16 +302,26 @@ struct DeviceArena {
     }
 };
 
+// Freed stream-ordered allocations stay in the device's default pool up to
+// this many bytes, so repeated runs of one grid reuse their memory without
+// new mappings; beyond it the pool returns memory to the device at the next
+// synchronisation, where other allocators (PyTorch's) can use it.
+constexpr uint64_t kPoolKeepBytes = 8ull << 30;
+
+bool default_pool(hipMemPool_t &pool)
+{
+    int dev = 0;
+    return hipGetDevice(&dev) == hipSuccess &&
+           hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess;
+}
+
 void keep_pool_memory()
 {
     static std::once_flag once;
     std::call_once(once, [] {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess)
-            return;
         hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-            uint64_t thr = UINT64_MAX;
+        if (default_pool(pool)) {
+            uint64_t thr = kPoolKeepBytes;
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
         }
     });
@@ -1514,7 +1572,11 @@ double run_host(const Args &a)
         auto swap = [&](size_t x, size_t y) { std::swap(phys[x], phys[y]); };
 
         auto uploads = [&](size_t idx) { return !up_at[idx].empty(); };
-        time_loop(a, pr, plan, 0, plan.size(), rec_dev, a.n0 - 1, kFlagOverlapReceivers, st,
+        // the receiver overlap where no other run or plan of the process
+        // holds it (see OverlapLease)
+        OverlapLease lease;
+        const unsigned hflags = lease.acquire(false) ? kFlagOverlapReceivers : 0u;
+        time_loop(a, pr, plan, 0, plan.size(), rec_dev, a.n0 - 1, hflags, st,
                   slot_ptr, before, after, swap, uploads);
         hip_check(hipStreamSynchronize(st), "time loop");
         const double tl1 = now_s();
@@ -1548,6 +1610,7 @@ double run_host(const Args &a)
 // shot needs on the device and runs its steps in order, in any number of
 // chunks, on any stream.
 struct Plan {
+    OverlapLease lease;          // held with kFlagOverlapReceivers (released last)
     Args a;
     unsigned flags;
     hipStream_t st;              // stream of creation; the arena frees on it
@@ -1649,6 +1712,8 @@ Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
     validate(a);
     keep_pool_memory();
     std::unique_ptr<Plan> p(new Plan(a, st, flags));
+    if (flags & kFlagOverlapReceivers)
+        p->lease.acquire(true);
     p->slots = plan_slots(a);
     for (size_t idx = 0; idx < p->slots.size(); idx++) {
         const StepSlots &s = p->slots[idx];
@@ -1916,7 +1981,20 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     for (int d = 0; d < SW_DIM; d++)
         if (dims[d] < (size_t)(3 * r + 1))
             fail("the adjoint needs at least 3r+1 points per axis");
+    OverlapLease lease;
+    if (flags & kFlagOverlapReceivers)
+        lease.acquire(true);
     keep_pool_memory();
+    struct PoolTrim {
+        // the sweep's levels and checkpoints are the largest allocations of
+        // the library: hand them back to the device once it has finished
+        ~PoolTrim()
+        {
+            hipMemPool_t pool;
+            if (default_pool(pool))
+                (void)hipMemPoolTrimTo(pool, kPoolKeepBytes);
+        }
+    } trim;
     DeviceArena arena(st);
     Prepared pr = prepare(a, true, 0u, arena, st);
     if (pr.halo_sources)
@@ -1961,6 +2039,9 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
         hipLaunchKernelGGL(sw::k_grad_flush, dim3(nb), dim3(256), 0, st, gacc, grad,
                            (long long)npts);
     };
+    // the shot starts from rest whatever the caller's slots hold, so that
+    // the all-levels and checkpointed sweeps see the same initial state
+    zero(a.u, 3 * npts);
     zero(seg, (K + 2) * npts);
     zero(ubar, 3 * npts);
     zero(gacc, npts);
@@ -2250,6 +2331,22 @@ SW_EXPORT double SW_SUFFIX(simwave_hip_gradient_)(SW_FORWARD_PARAMS, const real_
 SW_EXPORT const char *simwave_hip_last_error(void)
 {
     return g_error.c_str();
+}
+
+// Returns the freed memory the current device's default pool keeps for
+// reuse to the device (the library's stream-ordered allocations come from
+// that pool), so that a free-memory query afterwards sees it.  0, or -1.
+SW_EXPORT int simwave_hip_trim_pool(void)
+{
+    const double r = guarded([&] {
+        hipMemPool_t pool;
+        if (!default_pool(pool))
+            fail("no default memory pool on the current device");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hip_check(hipMemPoolTrimTo(pool, 0), "hipMemPoolTrimTo");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
 }
 
 SW_EXPORT int simwave_hip_last_timings(double *out, int n)

@@ -242,8 +242,18 @@ class DeviceForward:
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
         stream.wait_event(self._ready)
+        end = self.kwargs["end_timestep"]
+        if tm.saving_stride == 0:
+            # the last level a full run returns is level T in slot T % 3; a
+            # run cut at end_timestep E returns its level E (slot E % 3)
+            slots = self.tensors["u_full"][end % 3:end % 3 + 1]
+        elif end == tm.timesteps:
+            slots = tm.remove_time_halo_region(self.tensors["u_full"])
+        else:
+            raise ValueError("results() of a saving_stride run needs every time step "
+                             "(end_timestep %d < %d)" % (end, tm.timesteps))
         with torch.cuda.stream(stream):
-            u = sm.remove_halo_region(tm.remove_time_halo_region(self.tensors["u_full"]))
+            u = sm.remove_halo_region(slots)
             u = u.contiguous()
             rec = self.tensors["shot_record"]
             u_h = torch.empty(u.shape, dtype=u.dtype, pin_memory=True)

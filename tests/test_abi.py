@@ -35,7 +35,9 @@ def test_header_declares_every_variant():
                      "simwave_hip_gradient_"):
             assert base + vn in names
     for common in ("simwave_hip_last_error", "simwave_hip_last_timings",
-                   "simwave_hip_variant"):
+                   "simwave_hip_variant", "simwave_hip_trim_pool",
+                   "simwave_hip_overlap_acquire", "simwave_hip_overlap_release",
+                   "simwave_hip_overlap_holders"):
         assert common in names
 
 
@@ -117,3 +119,51 @@ def test_forward_rejects_bad_space_order():
     rc, err = _run_failing(3)
     assert rc < 0
     assert "space_order 3" in err
+
+
+_LEASE_SCRIPT = r"""
+import ctypes, sys
+sys.path.insert(0, %(root)r)
+sys.path.insert(0, %(tests)r)
+import cases
+import simwave_amd as sw
+from simwave_amd.middleware import Middleware
+a = ctypes.CDLL(sw.build.library_path(3, "cd", "f32"))
+b = ctypes.CDLL(sw.build.library_path(2, "vd", "f64"))
+# one lease per process, whichever variant library takes it
+assert a.simwave_hip_overlap_acquire() == 1
+assert b.simwave_hip_overlap_holders() == 1
+assert b.simwave_hip_overlap_acquire() == 0
+parts = cases.make_case(dim=2, n=24, tf=0.02)()
+kw = Middleware.normalize(**sw.Solver(**parts).forward_arguments())
+lib = ctypes.CDLL(sw.build.library_path(2, "cd", "f32"))
+argtypes, args = Middleware.ordered(kw)
+lib.simwave_hip_last_error.restype = ctypes.c_char_p
+for name in ("simwave_hip_forward_device_2d_cd_f32", "simwave_hip_plan_create_2d_cd_f32"):
+    fn = getattr(lib, name)
+    fn.argtypes = argtypes + [ctypes.c_void_p, ctypes.c_uint]
+    fn.restype = ctypes.c_double if "forward" in name else ctypes.c_void_p
+    r = fn(*args, None, 4)
+    print("CALL", name, r if r else 0, lib.simwave_hip_last_error().decode())
+    assert b.simwave_hip_overlap_holders() == 1
+b.simwave_hip_overlap_release()
+assert a.simwave_hip_overlap_holders() == 0
+print("DONE")
+"""
+
+
+def test_receiver_overlap_lease_is_process_wide():
+    """Flag 4 (receiver gather on a second stream) is refused while another
+    run or plan of the process holds it, across variant libraries, and a
+    refused call leaves the holder's lease in place."""
+    code = _LEASE_SCRIPT % dict(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    env = dict(os.environ)
+    env["HIP_VISIBLE_DEVICES"] = "-1"
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         timeout=300, env=env)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    calls = [line for line in res.stdout.splitlines() if line.startswith("CALL")]
+    assert len(calls) == 2
+    for line in calls:
+        assert "receiver overlap (flag 4) is already in use" in line, line
+    assert "DONE" in res.stdout
