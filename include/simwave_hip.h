@@ -125,8 +125,11 @@ int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -152,6 +155,12 @@ double simwave_hip_gradient_2d_cd_f32(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_2d_cd_f32(float b, float *y, float *lo, float *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_2d_cd_f32(float b);
 
 /* ---- 2d_cd_f64 */
 /* Host arrays, as the reference's forward(): reference constant_density/2d/wave.c:23-36
@@ -236,8 +245,11 @@ int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -263,6 +275,12 @@ double simwave_hip_gradient_2d_cd_f64(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_2d_cd_f64(double b, double *y, double *lo, double *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_2d_cd_f64(double b);
 
 /* ---- 2d_vd_f32 */
 /* Host arrays, as the reference's forward(): reference variable_density/2d/wave.c:23-36
@@ -347,8 +365,11 @@ int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -374,6 +395,12 @@ double simwave_hip_gradient_2d_vd_f32(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_2d_vd_f32(float b, float *y, float *lo, float *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_2d_vd_f32(float b);
 
 /* ---- 2d_vd_f64 */
 /* Host arrays, as the reference's forward(): reference variable_density/2d/wave.c:23-36
@@ -458,8 +485,11 @@ int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -485,6 +515,12 @@ double simwave_hip_gradient_2d_vd_f64(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_2d_vd_f64(double b, double *y, double *lo, double *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_2d_vd_f64(double b);
 
 /* ---- 3d_cd_f32 */
 /* Host arrays, as the reference's forward(): reference constant_density/3d/wave.c:23-36
@@ -569,8 +605,11 @@ int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -596,6 +635,12 @@ double simwave_hip_gradient_3d_cd_f32(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_3d_cd_f32(float b, float *y, float *lo, float *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_3d_cd_f32(float b);
 
 /* ---- 3d_cd_f64 */
 /* Host arrays, as the reference's forward(): reference constant_density/3d/wave.c:23-36
@@ -680,8 +725,11 @@ int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -707,6 +755,12 @@ double simwave_hip_gradient_3d_cd_f64(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_3d_cd_f64(double b, double *y, double *lo, double *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_3d_cd_f64(double b);
 
 /* ---- 3d_vd_f32 */
 /* Host arrays, as the reference's forward(): reference variable_density/3d/wave.c:23-36
@@ -791,8 +845,11 @@ int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -818,6 +875,12 @@ double simwave_hip_gradient_3d_vd_f32(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_3d_vd_f32(float b, float *y, float *lo, float *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_3d_vd_f32(float b);
 
 /* ---- 3d_vd_f64 */
 /* Host arrays, as the reference's forward(): reference variable_density/3d/wave.c:23-36
@@ -902,8 +965,11 @@ int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
  * forward states saved every that many steps and recomputed per segment
  * (>= the step count: every level kept in device memory, no recomputation).
  * flags: 1 sets grad_velocity to the zero-lag cross-correlation image
- * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 4 runs
- * the forward's receiver gather on a second stream (large grids).
+ * sum_k u^k lambda^k of the forward and adjoint wavefields instead; 2 takes
+ * rec_adjoint as observed data and drives the adjoint with the residual
+ * (shot record - rec_adjoint) formed on the device after the forward (the
+ * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
+ * on a second stream (large grids).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -929,6 +995,12 @@ double simwave_hip_gradient_3d_vd_f64(
  * spacing, cd-3d wave.c:186).  Returns 1 and y = RN(1/b) and the safe
  * numerator range [lo, hi], or 0 when IEEE division is used. */
 int simwave_hip_fixed_div_3d_vd_f64(double b, double *y, double *lo, double *hi);
+/* Host-only helper (no device work, no reference counterpart): 1 when the
+ * 3D tiled kernel forms the quotients sum/b of the divisor b as
+ * RN(sum * RN64(1/b)), the float64 product rounded to float (float32
+ * libraries, b = B 2^t with B odd, t <= 0, b in [2^-126, 2^24): equal to
+ * the IEEE quotient for every float sum); 0 otherwise. */
+int simwave_hip_f64_quotient_3d_vd_f64(double b);
 
 /* ---- every library */
 /* Message of the last failed call on this thread ("" after a success). */

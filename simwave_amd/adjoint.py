@@ -23,9 +23,11 @@ import numpy as np
 from .device import DeviceForward
 from .middleware import _last_error
 
-__all__ = ["gradient", "misfit_gradient", "image", "ShotRecord", "checkpoint_interval"]
+__all__ = ["gradient", "misfit_gradient", "shot_misfit_gradient", "image", "ShotRecord",
+           "checkpoint_interval"]
 
 IMAGE = 1     # adjoint entry flag: cross-correlation image instead of gradient
+RESIDUAL = 2  # adjoint entry flag: rec_adjoint is observed data, the source the residual
 OVERLAP_RECEIVERS = 4   # forward receiver gather on a second stream (large grids)
 
 
@@ -56,7 +58,7 @@ def checkpoint_interval(dev, checkpoint_every=None):
 
 
 def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=False,
-         overlap_receivers=True):
+         overlap_receivers=True, residual=False):
     import torch
     kw = dev.kwargs
     checkpoint_every = checkpoint_interval(dev, checkpoint_every)
@@ -84,7 +86,7 @@ def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=Fa
     secs = fn(*dev._args, ctypes.c_void_p(rec.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
               ctypes.c_void_p(gw.data_ptr() if gw is not None else 0),
               ctypes.c_size_t(int(checkpoint_every)), ctypes.c_void_p(stream.cuda_stream),
-              ctypes.c_uint((IMAGE if image else 0)
+              ctypes.c_uint((IMAGE if image else 0) | (RESIDUAL if residual else 0)
                             | (OVERLAP_RECEIVERS if overlap_receivers else 0)))
     if secs < 0:
         raise RuntimeError("adjoint failed: {}".format(_last_error(lib)))
@@ -140,15 +142,22 @@ def misfit_gradient(solver, observed, checkpoint_every=None, velocity=None, devi
     velocity (float64, extended shape)."""
     _check(solver)
     dev = DeviceForward(solver, device=device, velocity=velocity)
-    dev.run()
-    d = dev.tensors["shot_record"].double()
-    import torch
-    obs = torch.as_tensor(np.asarray(observed), device=dev.device).double()
-    res = d - obs
-    misfit = 0.5 * float((res * res).sum())
-    dev.reset()
-    grad, _ = _run(dev, res, checkpoint_every, False)
+    misfit, grad = shot_misfit_gradient(dev, observed, checkpoint_every)
     return misfit, grad.cpu().numpy()
+
+
+def shot_misfit_gradient(dev, observed, checkpoint_every=None, stream=None):
+    """One call of the adjoint entry in residual mode on a device-resident
+    shot: the forward, the residual d - observed on the device, the reverse
+    sweep.  Returns (misfit 1/2 ||d - observed||^2 as a float, gradient
+    with respect to the extended velocity as a float64 device tensor)."""
+    import torch
+    rec = dev.tensors["shot_record"]
+    obs = torch.as_tensor(observed, device=dev.device).to(rec.dtype).contiguous()
+    dev.reset()
+    grad, _ = _run(dev, obs, checkpoint_every, False, stream=stream, residual=True)
+    res = rec.double() - obs.double()
+    return 0.5 * float((res * res).sum()), grad
 
 
 @functools.lru_cache(maxsize=None)

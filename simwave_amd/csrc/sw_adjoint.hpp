@@ -643,6 +643,16 @@ __global__ void k_fill(real_t *p, long long n, real_t v)
         p[i] = v;
 }
 
+// r = d - o (the L2 misfit's adjoint source from the shot record d and the
+// observed data o)
+__global__ void k_residual(const real_t *__restrict__ d, const real_t *__restrict__ o,
+                           real_t *__restrict__ r, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        r[i] = d[i] - o[i];
+}
+
 __global__ void k_fill_f64(double *p, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;

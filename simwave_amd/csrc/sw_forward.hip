@@ -105,6 +105,9 @@ constexpr size_t kGraphSteps = 48;   // a multiple of the 3-slot cycle
 // adjoint entry: accumulate the zero-lag cross-correlation image of the
 // forward and adjoint wavefields instead of the velocity gradient
 constexpr unsigned kAdjImage = 1u;
+// adjoint entry: rec_adjoint holds observed data; the adjoint source is the
+// residual (shot record - observed), formed on the device after the forward
+constexpr unsigned kAdjResidual = 2u;
 // reverse steps whose gradient terms are summed in the working precision
 // before they are added to the float64 gradient
 constexpr size_t kGradFlush = 16;
@@ -346,6 +349,7 @@ struct Prepared {
     const Params *Pd = nullptr;   // device copy of P for the tiled kernel
     const real_t *qc = nullptr;   // (dt2/s)/den per point for the tiled kernel
     bool fastdiv = false;
+    bool div64 = false;          // float64-reciprocal quotients in the tiled kernel
     sw::FixedDiv dvz{}, dvx{}, dvy{};
     sw::Tiling tiling{};
     int tiled_blocks = 0;
@@ -604,7 +608,36 @@ bool make_fixed_div(real_t b, sw::FixedDiv &d)
     return true;
 }
 
+// Whether RN(a * RN64(1/b)), the product rounded to float64 and then to
+// float32, equals the IEEE float quotient a/b for every float a: b a
+// positive float in [2^-126, 2^24) whose odd significand B (b = B 2^t) has
+// t <= 0.  The float64 product is within 2^-52 (relative) of a/b.  A normal
+// quotient that is not a float lies at least 2^-49 from every midpoint
+// between floats (a - m b is a non-zero multiple of a unit at that scale,
+// as m b has more significant bits than a float holds).  A subnormal one
+// lies at least 2^-48 from the midpoints (2j+1) 2^-150: with t <= 0,
+// (2j+1) B 2^(t-150) is an odd multiple of 2^(t-150) and a is a multiple
+// of 2^-149, so they differ.  Both separations exceed the product's error.
+bool f64_quotient_exact(real_t b)
+{
+    if (sizeof(real_t) != 4)
+        return false;
+    const double bd = (double)b;
+    if (!(bd >= std::ldexp(1.0, -126) && bd < std::ldexp(1.0, 24)))
+        return false;
+    int e = 0;
+    const double m = std::frexp(bd, &e);              // bd = m 2^e, m in [0.5, 1)
+    unsigned long long S = (unsigned long long)std::ldexp(m, 24);   // 24-bit significand
+    int t = e - 24;
+    while (S && !(S & 1ull)) {
+        S >>= 1;
+        t++;
+    }
+    return t <= 0;
+}
+
 #if SW_DIM == 3
+
 // Fixed divisors for k_step3d_z's Laplacian quotients (IEEE ones when a
 // divisor has none or SIMWAVE_HIP_IEEE_DIV is set).
 void set_lap_div(Prepared &pr)
@@ -663,7 +696,7 @@ bool prefer_zmarch(const Prepared &pr)
     return R == 4 && interior < 50ll * 1000 * 1000;
 }
 
-template <int R, int WR, int RPT, int NW>
+template <int R, int WR, int RPT, int NW, bool D64>
 void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un,
                   int n, hipStream_t st)
 {
@@ -680,6 +713,9 @@ void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t
     A.dvy = pr.dvy;
     A.div_lo = std::max({pr.dvz.lo, pr.dvx.lo, pr.dvy.lo});
     A.div_hi = std::min({pr.dvz.hi, pr.dvx.hi, pr.dvy.hi});
+    A.ry = 1.0 / (double)P.dy2;
+    A.rx = 1.0 / (double)P.dx2;
+    A.rz = 1.0 / (double)P.dz2;
     A.nz = P.nz;
     A.nx = P.nx;
     A.ny = P.ny;
@@ -695,7 +731,7 @@ void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t
     A.nsrc = P.nsrc;
     A.sbox = P.sbox;
     A.T = pr.tiling;
-    hipLaunchKernelGGL((sw::k_step3d_vec<R, WR, RPT, NW>), dim3(pr.tiled_blocks),
+    hipLaunchKernelGGL((sw::k_step3d_vec<R, WR, RPT, NW, D64>), dim3(pr.tiled_blocks),
                        dim3(64, NW), 0, st, A, pr.Pd, n);
 }
 
@@ -703,6 +739,7 @@ using TiledFn = void (*)(const Prepared &, const real_t *, const real_t *, real_
                          hipStream_t);
 struct TiledKernel {
     TiledFn launch;
+    TiledFn launch64;   // float64-reciprocal quotients (float32 builds)
     int ty, txr;
 };
 
@@ -710,7 +747,9 @@ template <int R, int WR, int RPT, int NW>
 TiledKernel tiled_config()
 {
     using S = sw::TileShape<R, WR, RPT, NW>;
-    return TiledKernel{&tiled_launch<R, WR, RPT, NW>, S::TY, S::TXR};
+    constexpr bool f32 = sizeof(real_t) == 4;
+    return TiledKernel{&tiled_launch<R, WR, RPT, NW, false>,
+                       &tiled_launch<R, WR, RPT, NW, f32>, S::TY, S::TXR};
 }
 
 template <int R>
@@ -827,7 +866,10 @@ void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
     }
     pr.fastdiv = !env_flag("SIMWAVE_HIP_IEEE_DIV") && make_fixed_div(P.dz2, pr.dvz) &&
                  make_fixed_div(P.dx2, pr.dvx) && make_fixed_div(P.dy2, pr.dvy);
-    pr.tiled_launch = K.launch;
+    pr.div64 = sizeof(real_t) == 4 && !env_flag("SIMWAVE_HIP_IEEE_DIV") &&
+               !env_flag("SIMWAVE_HIP_NO_DIV64") && f64_quotient_exact(P.dz2) &&
+               f64_quotient_exact(P.dx2) && f64_quotient_exact(P.dy2);
+    pr.tiled_launch = pr.div64 ? K.launch64 : K.launch;
     pr.tiling = T;
     pr.tiled_blocks = (int)blocks;
     pr.tiled = true;
@@ -1973,8 +2015,9 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
         fail("NULL gradient array");
     if (a.nrec && !rec_adj)
         fail("NULL shot-record adjoint");
-    if (flags & ~(kAdjImage | kFlagOverlapReceivers))
-        fail("unknown adjoint flags 0x%x", flags & ~(kAdjImage | kFlagOverlapReceivers));
+    if (flags & ~(kAdjImage | kAdjResidual | kFlagOverlapReceivers))
+        fail("unknown adjoint flags 0x%x",
+             flags & ~(kAdjImage | kAdjResidual | kFlagOverlapReceivers));
     const bool image = flags & kAdjImage;
     const int r = (int)(a.space_order / 2);
     const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : (size_t)(3 * r + 1)};
@@ -2085,6 +2128,16 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     } else {
         time_loop(a, pr, plan, 0, plan.size(), a.receivers, 0, flags & kFlagOverlapReceivers,
                   st, slot_ptr, save, nop, noswap, no_uploads);
+    }
+    if ((flags & kAdjResidual) && a.nrec) {
+        // the shot record rows 0 .. T-1 are complete on `st`
+        real_t *res = arena.alloc<real_t>(T * a.nrec);
+        const long long n = (long long)(T * a.nrec);
+        hipLaunchKernelGGL(sw::k_residual,
+                           dim3((unsigned)std::min<long long>((n + 255) / 256, 16384)),
+                           dim3(256), 0, st, a.receivers, rec_adj, res, n);
+        hip_check(hipGetLastError(), "residual");
+        rec_adj = res;
     }
     const double t_fwd = now_s();
 
@@ -2369,6 +2422,14 @@ SW_EXPORT int SW_SUFFIX(simwave_hip_fixed_div_)(real_t b, real_t *y, real_t *lo,
     *lo = d.lo;
     *hi = d.hi;
     return 1;
+}
+
+// Host-only helper (no device work): 1 when the 3D tiled kernel of a float32
+// library forms the quotients sum/b as RN(sum * RN64(1/b)) for the divisor b
+// (see f64_quotient_exact), 0 otherwise.
+SW_EXPORT int SW_SUFFIX(simwave_hip_f64_quotient_)(real_t b)
+{
+    return f64_quotient_exact(b) ? 1 : 0;
 }
 
 SW_EXPORT const char *simwave_hip_variant(void)

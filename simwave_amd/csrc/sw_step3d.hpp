@@ -161,6 +161,26 @@ __device__ __forceinline__ real_t leapfrog_damped(real_t value, real_t C, real_t
     return (real_t)t;
 }
 
+// The same update in float32 with one correctly rounded float64 reciprocal
+// y = RN(1/den) for the two quotients: RN(2/den) = 2y exactly, and for float
+// operands num/den (a normal quotient), rounding num*y first to double and
+// then to float gives RN(num/den): the product is within 2^-52 of the
+// quotient, and a quotient of two floats that is not a float lies at least
+// 2^-49 (relative) from every midpoint between floats.
+__device__ __forceinline__ float leapfrog_damped_rcp(float value, float C, float e,
+                                                     float uc, float up)
+{
+    const float den = (float)(1.0 + (double)e);
+    const float num = (float)(1.0 - (double)e);
+    value = value * C;
+    const double y = 1.0 / (double)den;
+    const double a = 2.0 * y;
+    const float b = (float)((double)num * y);
+    double t = a * (double)uc - (double)(b * up);
+    t = t + (double)value;
+    return (float)t;
+}
+
 // Leap-frog update where den == num == 1 (no damping at the point; there
 // C == q).
 __device__ __forceinline__ real_t leapfrog_plain(real_t value, real_t C, real_t uc,
@@ -185,6 +205,7 @@ struct StepArgs {
     real_t dz2, dx2, dy2;
     FixedDiv dvz, dvx, dvy;
     real_t div_lo, div_hi;  // numerator range safe for all three divisors
+    double ry, rx, rz;      // RN(1/dy2) ... in float64 (quotient mode D64)
     real_t c2[kMaxR + 1];
     int nsrc;
     const Box *sbox;
@@ -207,7 +228,14 @@ struct TileShape {
     static constexpr int HY = (NYV + 64 * NW - 1) / (64 * NW);
 };
 
-template <int R, int WR, int RPT, int NW>
+// D64 (float32 only): the Laplacian quotients sum/h2 as RN(sum * RN(1/h2)),
+// the product in float64; equal to the IEEE float quotient for every float
+// sum when h2 = B 2^t (B odd) has t <= 0 and lies in [2^-126, 2^24), which
+// the host checks (the product is within 2^-52 of the quotient, every
+// non-float quotient at least 2^-49 from a rounding midpoint, and with t <= 0
+// no quotient in the subnormal range lands on a midpoint).  Otherwise the
+// fixed-divisor float quotients with their range vote.
+template <int R, int WR, int RPT, int NW, bool D64 = false>
 __global__ void __launch_bounds__(64 * NW)
 k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
 {
@@ -501,10 +529,17 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                     acc_z[e] += A.c2[m] * (qz[q][R + m][e] + qz[q][R - m][e]);
                 }
             }
+            real_t lap[V];
+            if constexpr (D64) {
+#pragma unroll
+                for (int e = 0; e < V; e++)
+                    lap[e] = (real_t)((double)acc_y[e] * A.ry) +
+                             (real_t)((double)acc_x[e] * A.rx) +
+                             (real_t)((double)acc_z[e] * A.rz);
+            } else {
             // Fixed-divisor quotients for every point; the IEEE quotients
             // replace them where a numerator is outside the safe range (or
             // zero next to a non-zero one), decided once per wavefront.
-            real_t lap[V];
             bool ok = true;
 #pragma unroll
             for (int e = 0; e < V; e++) {
@@ -526,13 +561,20 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                         lap[e] = ay / A.dy2 + ax / A.dx2 + az / A.dz2;
                 }
             }
+            }
 #pragma unroll
             for (int e = 0; e < V; e++) {
                 real_t value = 0;
                 value += lap[e];
+#if defined(SW_DOUBLE)
                 out[q][e] = damped ? leapfrog_damped(value, nq[q][e], ne[q][e], qz[q][R][e],
                                                      np[q][e])
                                    : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
+#else
+                out[q][e] = damped ? leapfrog_damped_rcp(value, nq[q][e], ne[q][e],
+                                                         qz[q][R][e], np[q][e])
+                                   : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
+#endif
             }
             if (!CLEAN && special && act[q]) {
                 // tiles touching a source box or a boundary face

@@ -1,0 +1,247 @@
+"""Synthetic experiments at the sizes the benchmark and the size tests use.
+
+Every builder returns the keyword arguments of ``Solver`` (space_model,
+time_model, sources, receivers, wavelet) with seeded synthetic models of
+the stated structure (there are no field datasets here):
+
+* ``readme_2d``: the reference README example (README.md:54-137),
+  2D constant density, space order 4, 512^2 resampled to 513^2;
+* ``headline_3d``: 3D constant density, 384^3 at 12.5 m, space order 8,
+  40-cell damping, 6 layers with smooth +-5% noise, tf 1.5 s, 128 x 128
+  receivers;
+* ``faulted_2d_vd``: 2D variable density, 1000 x 3000 at 5 m, space
+  order 16, 60-cell damping, dipping faulted layers, Gardner density,
+  tf 3.0 s, 3000 receivers, saving_stride 10;
+* ``fwi_models`` / ``fwi_shot``: the FWI experiment, 160 x 256 x 256 at
+  15 m, space order 8, 30-cell damping, a layered true model with a
+  spherical +10% anomaly, its sigma-8 Gaussian smoothing as the starting
+  model, 16 sources on a 4 x 4 surface grid, 100 x 100 receivers;
+* ``serving_requests``: a seeded stream of mixed 2D/3D requests.
+"""
+import numpy as np
+
+__all__ = ["readme_2d", "headline_3d", "faulted_2d_vd", "fwi_models", "fwi_shot",
+           "serving_requests", "interior_updates"]
+
+
+def interior_updates(solver_or_kwargs):
+    """Interior grid-point updates of one run (points x time steps)."""
+    kw = solver_or_kwargs
+    if hasattr(kw, "forward_arguments"):
+        sm, tm = kw.space_model, kw.time_model
+        r = sm.space_order // 2
+        return int(np.prod([n - 2 * r for n in sm.extended_shape])) * int(tm.timesteps)
+    r = kw["space_order"] // 2
+    shape = kw["velocity_model"].shape
+    return int(np.prod([n - 2 * r for n in shape])) * int(kw["end_timestep"])
+
+
+def readme_2d(dtype=np.float32):
+    """The reference README example: 512^2 -> 513^2, so4, 512 receivers."""
+    import simwave_amd as sw
+    vel = np.full((512, 512), 1500.0, dtype=dtype)
+    vel[100:] = 2000.0
+    sm = sw.SpaceModel(bounding_box=(0, 5120, 0, 5120), grid_spacing=(10, 10),
+                       velocity_model=vel, space_order=4, dtype=dtype)
+    sm.config_boundary(damping_length=0,
+                       boundary_condition=("null_neumann", "null_dirichlet",
+                                           "none", "null_dirichlet"))
+    tm = sw.TimeModel(space_model=sm, tf=1.0, saving_stride=0)
+    src = sw.Source(sm, coordinates=[(2560, 2560)], window_radius=1)
+    rec = sw.Receiver(sm, coordinates=[(2560, i) for i in range(0, 5120, 10)],
+                      window_radius=1)
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(10.0, tm))
+
+
+def headline_3d(dtype=np.float32, seed=0, tf=1.5):
+    """384^3 at 12.5 m, so8, 40-cell cubic damping (alpha 0.001); 6 flat
+    layers 1500..4500 m/s times (1 + p), p seeded Gaussian-smoothed noise
+    (sigma 8 cells) scaled to max |p| = 5%; one 15 Hz Ricker at
+    (100, 2393.75, 2393.75) m; 128 x 128 receivers at z = 25 m; BCs
+    (Neumann at the surface, Dirichlet elsewhere); saving_stride 0."""
+    import simwave_amd as sw
+    from scipy.ndimage import gaussian_filter
+
+    n, h = 384, 12.5
+    rng = np.random.default_rng(seed)
+    noise = gaussian_filter(rng.standard_normal((n, n, n), dtype=np.float32),
+                            8.0, mode="reflect", truncate=3.0)
+    noise *= np.float32(0.05 / np.abs(noise).max())
+    layers = np.linspace(1500.0, 4500.0, 6)
+    vel = np.repeat(layers, n // 6).astype(np.float32)[:, None, None] * (1 + noise)
+    vel = vel.astype(dtype)
+    del noise
+    ext = (n - 1) * h
+    sm = sw.SpaceModel(bounding_box=(0, ext) * 3, grid_spacing=(h, h, h),
+                       velocity_model=vel, space_order=8, dtype=dtype)
+    sm.config_boundary(damping_length=500.0,
+                       boundary_condition=("null_neumann",) + ("null_dirichlet",) * 5,
+                       damping_polynomial_degree=3, damping_alpha=0.001)
+    tm = sw.TimeModel(space_model=sm, tf=tf, saving_stride=0)
+    src = sw.Source(sm, coordinates=[(100.0, ext / 2, ext / 2)])
+    g = np.linspace(0.0, ext, 128)
+    rec = sw.Receiver(sm, coordinates=[(25.0, x, y) for x in g for y in g])
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(15.0, tm))
+
+
+def faulted_2d_vd(dtype=np.float32, seed=0, tf=3.0):
+    """2D variable density, 1000 x 3000 at 5 m, space order 16, 60-cell
+    damping; 20 seeded dipping layers (1500-5500 m/s) offset by 3 seeded
+    normal faults; density from Gardner's relation rho = 310 v^0.25;
+    25 Hz Ricker at the surface centre; 3000 surface receivers;
+    saving_stride 10 (every 10th wavefield returned)."""
+    import simwave_amd as sw
+    nz, nx, h = 1000, 3000, 5.0
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[:, None] * h
+    x = np.arange(nx)[None, :] * h
+    dip = np.tan(np.deg2rad(rng.uniform(-8.0, 8.0)))
+    depth = z - dip * (x - x.mean())
+    for _ in range(3):                      # normal faults: the hanging wall drops
+        x0 = rng.uniform(0.2, 0.8) * nx * h
+        slope = np.tan(np.deg2rad(rng.uniform(55.0, 75.0)))
+        throw = rng.uniform(40.0, 150.0)
+        hanging = x > x0 + z / slope
+        depth = np.where(hanging, depth - throw, depth)
+    edges = np.sort(rng.uniform(0.0, nz * h, 19))
+    speeds = np.linspace(1500.0, 5500.0, 20)
+    vel = speeds[np.searchsorted(edges, depth)].astype(dtype)
+    den = (310.0 * vel.astype(np.float64) ** 0.25).astype(dtype)
+    sm = sw.SpaceModel(bounding_box=(0, (nz - 1) * h, 0, (nx - 1) * h),
+                       grid_spacing=(h, h), velocity_model=vel, density_model=den,
+                       space_order=16, dtype=dtype)
+    sm.config_boundary(damping_length=60 * h,
+                       boundary_condition=("null_neumann",) + ("null_dirichlet",) * 3,
+                       damping_polynomial_degree=3, damping_alpha=0.001)
+    tm = sw.TimeModel(space_model=sm, tf=tf, saving_stride=10)
+    src = sw.Source(sm, coordinates=[(2 * h, (nx - 1) * h / 2)])
+    rec = sw.Receiver(sm, coordinates=[(2 * h, x) for x in
+                                       np.linspace(0.0, (nx - 1) * h, 3000)])
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(25.0, tm))
+
+
+# FWI experiment geometry
+FWI_SHAPE = (160, 256, 256)
+FWI_SPACING = 15.0
+FWI_SHOTS = 16
+
+
+def fwi_models(seed=0):
+    """(true, initial) velocity of the FWI experiment, float64, shape
+    160 x 256 x 256: 5 flat layers 1500..4000 m/s with a seeded sphere
+    (radius 20-30 cells, centre in the middle fifth of each axis) 10%
+    faster; the initial model is the true one Gaussian-smoothed with
+    sigma 8 cells."""
+    from scipy.ndimage import gaussian_filter
+    nz, nx, ny = FWI_SHAPE
+    rng = np.random.default_rng(seed)
+    layers = np.linspace(1500.0, 4000.0, 5)
+    true = np.repeat(layers, nz // 5)[:, None, None] * np.ones((1, nx, ny))
+    c = [rng.uniform(0.4, 0.6) * n for n in FWI_SHAPE]
+    rad = rng.uniform(20.0, 30.0)
+    zz, xx, yy = np.meshgrid(*[np.arange(n) for n in FWI_SHAPE], indexing="ij",
+                             sparse=True)
+    inside = (zz - c[0]) ** 2 + (xx - c[1]) ** 2 + (yy - c[2]) ** 2 <= rad ** 2
+    true = np.where(inside, 1.1 * true, true)
+    init = gaussian_filter(true, 8.0, mode="nearest")
+    return true, init
+
+
+def fwi_shot(vel, shot, dtype=np.float32, dt=None, tf=1.2):
+    """Shot `shot` (0..15) of the FWI experiment over the velocity `vel`
+    (160 x 256 x 256): so8, 30-cell cubic damping (228 x 324 x 324
+    extended), Neumann surface, Dirichlet elsewhere; a 10 Hz Ricker at
+    30 m depth on a 4 x 4 grid; 100 x 100 receivers at 15 m depth; tf
+    1.2 s; time step `dt` (default: the model's CFL step; the observed and
+    modelled data of an FWI share the true model's)."""
+    import simwave_amd as sw
+    nz, nx, ny = FWI_SHAPE
+    h = FWI_SPACING
+    ez, ex, ey = (nz - 1) * h, (nx - 1) * h, (ny - 1) * h
+    sm = sw.SpaceModel(bounding_box=(0, ez, 0, ex, 0, ey), grid_spacing=(h, h, h),
+                       velocity_model=np.asarray(vel, dtype=dtype), space_order=8,
+                       dtype=dtype)
+    sm.config_boundary(damping_length=30 * h,
+                       boundary_condition=("null_neumann",) + ("null_dirichlet",) * 5,
+                       damping_polynomial_degree=3, damping_alpha=0.001)
+    tm = sw.TimeModel(space_model=sm, tf=tf, saving_stride=0, dt=dt)
+    sx = ex * (0.5 + (shot % 4)) / 4
+    sy = ey * (0.5 + (shot // 4 % 4)) / 4
+    src = sw.Source(sm, coordinates=[(30.0, sx, sy)])
+    g = np.linspace(0.0, ex, 100)
+    rec = sw.Receiver(sm, coordinates=[(15.0, x, y) for x in g for y in g])
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(10.0, tm))
+
+
+def fwi_dt(true_vel, dtype=np.float32):
+    """CFL time step of the true model (the larger velocity bound), shared
+    by the observed and modelled data."""
+    return float(fwi_shot(true_vel, 0, dtype=dtype)["time_model"].dt)
+
+
+def serving_requests(count, seed=0, n2=(256, 2048), n3=(128, 384), steps=None,
+                     tf_scale=1.0):
+    """Seeded stream of shot requests: 2D or 3D (50/50), 2D grids
+    n2[0]^2..n2[1]^2 and 3D grids n3[0]^3..n3[1]^3 at 10 m, space order
+    uniform in {2,4,8,12,16,20}, constant or variable density 50/50
+    (Gardner), 3-8 random flat layers of 1500-5000 m/s, a 5-40 Hz Ricker at
+    the shallow centre, 64-4096 surface receivers, 20-cell damping, float32.
+    Duration: with `steps` = (lo, hi), lo..hi time steps; otherwise
+    tf = tf_scale x the time the slowest layer's wave takes to cross the
+    grid's depth once.
+
+    Returns (requests, build): the request descriptions and a function that
+    builds a request's Solver."""
+    import simwave_amd as sw
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        dim = 2 if rng.random() < 0.5 else 3
+        n = int(rng.integers(*(n2 if dim == 2 else n3), endpoint=True))
+        so = int(rng.choice([2, 4, 8, 12, 16, 20]))
+        dens = bool(rng.random() < 0.5)
+        layers = np.sort(rng.uniform(1500.0, 5000.0, int(rng.integers(3, 8, endpoint=True))))
+        f0 = float(rng.uniform(5.0, 40.0))
+        nrec = int(rng.integers(64, 4096, endpoint=True))
+        nsteps = int(rng.integers(*steps, endpoint=True)) if steps else None
+        out.append(dict(dim=dim, n=n, so=so, dens=dens, layers=layers, f0=f0,
+                        nrec=nrec, nsteps=nsteps))
+
+    def build(req):
+        dim, n, h = req["dim"], req["n"], 10.0
+        shape = (n,) * dim
+        nl = len(req["layers"])
+        vel = np.repeat(req["layers"], -(-n // nl))[:n].astype(np.float32)
+        vel = np.broadcast_to(vel.reshape((n,) + (1,) * (dim - 1)), shape).copy()
+        den = (310.0 * vel.astype(np.float64) ** 0.25).astype(np.float32) \
+            if req["dens"] else None
+        ext = (n - 1) * h
+        sm = sw.SpaceModel(bounding_box=(0, ext) * dim, grid_spacing=(h,) * dim,
+                           velocity_model=vel, density_model=den,
+                           space_order=req["so"], dtype=np.float32)
+        sm.config_boundary(damping_length=20 * h,
+                           boundary_condition=("null_neumann",)
+                           + ("null_dirichlet",) * (2 * dim - 1))
+        if req["nsteps"]:
+            dt = sw.TimeModel(space_model=sm, tf=1.0).dt
+            tm = sw.TimeModel(space_model=sm, tf=float(dt) * (req["nsteps"] - 1))
+        else:
+            tm = sw.TimeModel(space_model=sm,
+                              tf=tf_scale * ext / float(req["layers"][0]))
+        src = sw.Source(sm, coordinates=[(2 * h,) + (ext / 2,) * (dim - 1)])
+        if dim == 2:
+            xs = np.linspace(0.0, ext, req["nrec"])
+            rc = [(h, x) for x in xs]
+        else:
+            m = int(np.ceil(np.sqrt(req["nrec"])))
+            g = np.linspace(0.0, ext, m)
+            rc = [(h, x, y) for x in g for y in g][:req["nrec"]]
+        rec = sw.Receiver(sm, coordinates=rc)
+        return sw.Solver(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                         wavelet=sw.RickerWavelet(req["f0"], tm))
+
+    return out, build

@@ -32,7 +32,7 @@ def test_header_declares_every_variant():
         for base in ("simwave_hip_forward_", "simwave_hip_forward_device_",
                      "simwave_hip_fixed_div_", "simwave_hip_plan_create_",
                      "simwave_hip_plan_run_", "simwave_hip_plan_destroy_",
-                     "simwave_hip_gradient_"):
+                     "simwave_hip_gradient_", "simwave_hip_f64_quotient_"):
             assert base + vn in names
     for common in ("simwave_hip_last_error", "simwave_hip_last_timings",
                    "simwave_hip_variant", "simwave_hip_trim_pool",

@@ -123,3 +123,83 @@ def test_float64_random(h):
         assert Fraction(r) == Fraction(av) - Fraction(q0) * fb
         q1 = float(Fraction(q0) + Fraction(r) * Fraction(y))  # one rounding: fma
         assert q1 == av / b
+
+
+F64_CHECKER = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+/* usage: checker b ; compares (float)((double)a * (1.0 / (double)b)) with the
+   IEEE quotient a / b for every float a with a subnormal or zero quotient
+   (every a below 2^-126 b) and every significand at a few normal exponents;
+   prints the mismatch count */
+static float fl(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+int main(int argc, char **argv) {
+    volatile float b = strtof(argv[1], 0);
+    const double y = 1.0 / (double)b;
+    int eb; frexpf(b, &eb);
+    long bad = 0;
+    /* biased exponents 0 (subnormal a) .. those whose quotient is subnormal,
+       then 1.0 and 2^60 */
+    int hi_exp = 126 + eb + 1;
+    if (hi_exp > 254) hi_exp = 254;
+    for (int be = 0; be <= hi_exp + 2; be++) {
+        int e = be <= hi_exp ? be : (be == hi_exp + 1 ? 127 : 187);
+        for (uint32_t m = 0; m < (1u << 23); m++) {
+            for (uint32_t sgn = 0; sgn < 2; sgn++) {
+                float a = fl((sgn << 31) | ((uint32_t)e << 23) | m);
+                volatile float ref = a / b;
+                float got = (float)((double)a * y);
+                if (bits(ref) != bits(got)) bad++;
+            }
+        }
+    }
+    printf("%ld\n", bad);
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def f64_checker(tmp_path_factory):
+    d = tmp_path_factory.mktemp("f64div")
+    src, exe = d / "c.c", d / "c"
+    src.write_text(F64_CHECKER)
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"],
+                   check=True)
+    return str(exe)
+
+
+def _f64_rule(b):
+    lib = ctypes.CDLL(build.library_path(3, "cd", "f32"))
+    fn = lib.simwave_hip_f64_quotient_3d_cd_f32
+    fn.argtypes = [ctypes.c_float]
+    fn.restype = ctypes.c_int
+    return fn(b) == 1
+
+
+@pytest.mark.skipif(not os.path.exists(build.library_path(3, "cd", "f32")),
+                    reason="HIP libraries not built")
+@pytest.mark.parametrize("b", [156.25, 225.0, float(np.float32(12.3456) ** 2)])
+def test_f64_reciprocal_quotient_is_exact(f64_checker, b):
+    """Divisors the library accepts for the float64-reciprocal quotients
+    (h2 = B 2^t with t <= 0: 12.5 m, 15 m and an arbitrary spacing)
+    give the IEEE quotient for every float numerator whose quotient is
+    subnormal or zero, and for every significand at two normal exponents."""
+    assert _f64_rule(b)
+    out = subprocess.run([f64_checker, repr(b)], capture_output=True, text=True,
+                         check=True, timeout=600)
+    assert int(out.stdout) == 0
+
+
+@pytest.mark.skipif(not os.path.exists(build.library_path(3, "cd", "f32")),
+                    reason="HIP libraries not built")
+def test_f64_reciprocal_rule_rejects_even_divisors():
+    """h2 = 100 (10 m), 6, 2^25 and subnormal divisors keep the
+    fixed-divisor float quotients: with t >= 1 a subnormal quotient can be an
+    exact midpoint, which the float64 product need not round to."""
+    for b in (100.0, 6.0, 2.0 ** 25, 1e-40):
+        assert not _f64_rule(b), b
