@@ -245,7 +245,11 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     constexpr int NYV = S::NYV, HY = S::HY;
     constexpr int NT = 64 * NW;
     constexpr unsigned esz = sizeof(real_t);
-    __shared__ __align__(16) real_t lds[2][H][LS];
+    // two plane buffers, each followed by one dump slot per thread that
+    // takes the halo writes of lanes without a halo vector, so that every
+    // lane issues the same LDS stores (no branches around them)
+    constexpr int DUMP = H * LS;
+    __shared__ __align__(16) real_t lds[2][H * LS + NT * V];
 
     const int lane = threadIdx.x, w = threadIdx.y;
     const int tid = w * 64 + lane;
@@ -301,14 +305,15 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     const bool full = k + V <= ny - R;      // every point of the vector is interior
     // x halo: rows j0-R .. j0-1 and j0+TXR .. j0+TXR+R-1
     unsigned xh_off[HX];
-    int xh_row[HX];
+    int xh_lds[HX];
 #pragma unroll
     for (int h = 0; h < HX; h++) {
         const int hr = slot + SLOTS * h;
         const int jr = hr < R ? j0 - R + hr : j0 + TXR + (hr - R);
         const int jl = jr < nx ? jr : nx - 1;
         xh_off[h] = ((unsigned)jl * (unsigned)ny + (unsigned)kl) * esz;
-        xh_row[h] = hr < 2 * R ? (hr < R ? hr : hr + TXR) : -1;
+        const int xr = hr < 2 * R ? (hr < R ? hr : hr + TXR) : -1;
+        xh_lds[h] = xr >= 0 ? xr * LS + PADL + lc * V : DUMP + tid * V;
     }
     // y halo: HV vectors left of k0 and right of k0+TY on every tile row
     unsigned yh_off[HY];
@@ -324,7 +329,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
         // kr > -V and jl >= R >= 1 with ny >= V, so the offset is not negative
         yh_off[h] = (unsigned)((int)((unsigned)jl * (unsigned)ny) + kr) * esz;
         yh_lds[h] = idx < NYV ? (R + row) * LS + (c < HV ? V * c : PADL + TY + V * (c - HV))
-                              : -1;
+                              : DUMP + tid * V;
     }
 
     // tile-uniform boundary and source tests
@@ -370,10 +375,10 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
 #pragma unroll
         for (int q = 0; q < RPT; q++)
             qz[q][m + 1] = vload(r_uc, cb[q] + (unsigned)m * PB, 0u);
-    // One register set per prefetched quantity: the centre plane and its
-    // halos for plane i+1 are loaded right after plane i is written to LDS;
-    // u_prev, q and the damping term for plane i+1 right after plane i's
-    // update has consumed them.
+    // One register set per prefetched quantity: the centre plane, its halos,
+    // u_prev, C and the damping term of plane i+1 are all loaded right after
+    // plane i is written to LDS (plane i's update operands move to their own
+    // registers first).
     vreal nc[RPT], np[RPT], nq[RPT], ne[RPT], nhx[HX], nhy[HY];
     const unsigned long long dbits = damp_bits(T, col, z0);
     auto load_centre = [&](int I) {
@@ -429,13 +434,21 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
             ne[q] = vload(r_e, damp ? cb[q] : kOOB, sp);
         }
     };
+    // The scheduler barriers keep the prologue's loads in the loop's order
+    // (centre and halos before the update operands), so the wait counts
+    // merged at the loop header stay exact: the halo writes of a plane then
+    // wait only for the halo loads, not for the update operands issued
+    // after them.
     if constexpr (CLEAN) {
         load_centre_clean(z0);
+        __builtin_amdgcn_sched_barrier(0);
         load_update_clean(z0);
     } else {
         load_centre(z0);
+        __builtin_amdgcn_sched_barrier(0);
         load_update(z0);
     }
+    __builtin_amdgcn_sched_barrier(0);
 
     // store offsets of clean tiles: rows past the interior and lanes
     // outside it write nothing
@@ -444,7 +457,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     for (int q = 0; q < RPT; q++)
         st_off[q] = (act[q] && full) ? cb[q] : kOOB;
     for (int i = z0; i < z1; i++) {
-        real_t *lb = &lds[(i - z0) & 1][0][0];
+        real_t *lb = &lds[(i - z0) & 1][0];
         const bool damped = (dbits >> (i - z0)) & 1;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
@@ -456,17 +469,28 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
         }
 #pragma unroll
         for (int h = 0; h < HX; h++)
-            if (xh_row[h] >= 0)
-                *(vreal *)&lb[xh_row[h] * LS + PADL + lc * V] = nhx[h];
+            *(vreal *)&lb[xh_lds[h]] = nhx[h];
 #pragma unroll
         for (int h = 0; h < HY; h++)
-            if (yh_lds[h] >= 0)
-                *(vreal *)&lb[yh_lds[h]] = nhy[h];
+            *(vreal *)&lb[yh_lds[h]] = nhy[h];
+        // this plane's update operands (loaded during the previous plane),
+        // then every load of the next plane, so that each load has a whole
+        // plane's computation to arrive
+        vreal cp[RPT], cq[RPT], ce[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            cp[q] = np[q];
+            cq[q] = nq[q];
+            ce[q] = ne[q];
+        }
         const bool more = i + 1 < z1;
-        if constexpr (CLEAN)
+        if constexpr (CLEAN) {
             load_centre_clean(i + 1);
-        else if (more)
+            load_update_clean(i + 1);
+        } else if (more) {
             load_centre(i + 1);
+            load_update(i + 1);
+        }
         __syncthreads();
 
         const bool bc_here =
@@ -567,13 +591,13 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                 real_t value = 0;
                 value += lap[e];
 #if defined(SW_DOUBLE)
-                out[q][e] = damped ? leapfrog_damped(value, nq[q][e], ne[q][e], qz[q][R][e],
-                                                     np[q][e])
-                                   : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
+                out[q][e] = damped ? leapfrog_damped(value, cq[q][e], ce[q][e], qz[q][R][e],
+                                                     cp[q][e])
+                                   : leapfrog_plain(value, cq[q][e], qz[q][R][e], cp[q][e]);
 #else
-                out[q][e] = damped ? leapfrog_damped_rcp(value, nq[q][e], ne[q][e],
-                                                         qz[q][R][e], np[q][e])
-                                   : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
+                out[q][e] = damped ? leapfrog_damped_rcp(value, cq[q][e], ce[q][e],
+                                                         qz[q][R][e], cp[q][e])
+                                   : leapfrog_plain(value, cq[q][e], qz[q][R][e], cp[q][e]);
 #endif
             }
             if (!CLEAN && special && act[q]) {
@@ -596,13 +620,10 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                 }
             }
         }
-        // stores first, then the next plane's update operands, so that
-        // waiting for those never waits for the stores
         if constexpr (CLEAN) {
 #pragma unroll
             for (int q = 0; q < RPT; q++)
                 vstore(out[q], r_un, st_off[q], so);
-            load_update_clean(i + 1);
         } else {
             if (!special) {
 #pragma unroll
@@ -619,8 +640,6 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                     }
                 }
             }
-            if (more)
-                load_update(i + 1);
         }
     }
     };

@@ -101,6 +101,16 @@ CASES = {
                         dtype=F64, tf=0.03),
     "3d_deep_so4": dict(dim=3, shape=(52, 30, 260), space_order=4, damping=30.0,
                         tf=0.04),
+    # spacings whose squares B 2^t (B odd) have t <= 0 (12.5^2 = 625/4,
+    # 15^2 = 225): the float32 tiled kernel forms the Laplacian quotients
+    # as float64 products with RN(1/h^2)
+    "3d_deep_h12p5": dict(dim=3, shape=(60, 30, 301), h=12.5, space_order=8,
+                          damping=37.5, tf=0.05),
+    "3d_wide_h15_bc": dict(dim=3, shape=(25, 29, 275), h=15.0, space_order=8, damping=0.0,
+                           bc=("null_neumann", "null_dirichlet", "null_neumann",
+                               "null_neumann", "null_neumann", "null_neumann"),
+                           src=[(0.0, 150.0, 4050.0), (90.0, 210.0, 1950.0)], multi=True,
+                           tf=0.06),
 }
 
 
@@ -140,7 +150,7 @@ def test_parity_generic_path(gpu, name):
 # Tile configurations of the 3D tiled kernel (SIMWAVE_HIP_TILE): radius 4
 # carries 0..16, other radii 0, 4 and 8; SIMWAVE_HIP_ORDER picks the tile
 # order.
-TILED_R4 = ["3d_cd_so8", "3d_wide_so8", "3d_wide_f64_so8", "3d_wide_bc"]
+TILED_R4 = ["3d_cd_so8", "3d_wide_so8", "3d_wide_f64_so8", "3d_wide_bc", "3d_wide_h15_bc"]
 TILED_OTHER = ["3d_cd_so2", "3d_cd_so20", "3d_bc_mixed", "3d_src_halo", "3d_rect",
                "3d_wide_so6"]
 
@@ -156,7 +166,7 @@ def _tiled(name, tile, order):
         os.environ.pop("SIMWAVE_HIP_ORDER", None)
 
 
-DEEP_R4 = ["3d_deep_so8", "3d_deep_so8_whole", "3d_deep_none", "3d_deep_f64"]
+DEEP_R4 = ["3d_deep_so8", "3d_deep_so8_whole", "3d_deep_none", "3d_deep_f64", "3d_deep_h12p5"]
 
 
 @pytest.mark.parametrize("order", [0, 1])
