@@ -143,6 +143,7 @@ class ShotServer:
         self._used = 0.0
         self._closed = False
         self._workers = []
+        self.admissions = []          # requests (as submitted) in admission order
         self.stats = ServeStats()
         self._sched = threading.Thread(target=self._schedule, name="shot-scheduler",
                                        daemon=True)
@@ -215,6 +216,7 @@ class ShotServer:
                         self._active >= self.max_active:
                     self._cv.wait()
                 self._waiting.popleft()
+                self.admissions.append(req.solver)
                 self._used += need
                 self._active += 1
                 active_now = self._active
