@@ -15,6 +15,8 @@
  *     pointers for the arrays that live on the GPU across calls;
  *   - simwave_hip_plan_{create,run,destroy}_<variant>: the device entry
  *     split into chunks for serving loops;
+ *   - simwave_hip_forward_device_noalloc_<variant>: the device entry with
+ *     every library buffer in a static device array (no allocation);
  *   - simwave_hip_fixed_div_<variant>, simwave_hip_last_error,
  *     simwave_hip_last_timings, simwave_hip_variant, simwave_hip_trim_pool;
  *   - through libsimwave_hip_common.so, which every variant library links:
@@ -84,6 +86,35 @@ double simwave_hip_forward_device_2d_cd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_2d_cd_f32, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of float: 8 bytes a grid
+ * point, so at most about 8.3 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_2d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_2d_cd_f32
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_2d_cd_f32(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or
@@ -204,6 +235,35 @@ double simwave_hip_forward_device_2d_cd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_2d_cd_f64, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of double: 16 bytes a grid
+ * point, so at most about 4.1 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_2d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_2d_cd_f64
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_2d_cd_f64(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or
@@ -324,6 +384,35 @@ double simwave_hip_forward_device_2d_vd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_2d_vd_f32, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of float: 8 bytes a grid
+ * point, so at most about 8.3 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_2d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, float dz, float dx,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_2d_vd_f32
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_2d_vd_f32(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or
@@ -444,6 +533,35 @@ double simwave_hip_forward_device_2d_vd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_2d_vd_f64, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of double: 16 bytes a grid
+ * point, so at most about 4.1 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_2d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, double dz, double dx,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_2d_vd_f64
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_2d_vd_f64(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or
@@ -564,6 +682,35 @@ double simwave_hip_forward_device_3d_cd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_3d_cd_f32, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of float: 12 bytes a grid
+ * point, so at most about 5.5 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_3d_cd_f32(
+    float *u, float *velocity, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_3d_cd_f32
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_3d_cd_f32(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or
@@ -684,6 +831,35 @@ double simwave_hip_forward_device_3d_cd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_3d_cd_f64, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of double: 24 bytes a grid
+ * point, so at most about 2.7 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_3d_cd_f64(
+    double *u, double *velocity, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_3d_cd_f64
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_3d_cd_f64(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or
@@ -804,6 +980,35 @@ double simwave_hip_forward_device_3d_vd_f32(
     size_t saving_stride, float dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_3d_vd_f32, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of float: 12 bytes a grid
+ * point, so at most about 5.5 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_3d_vd_f32(
+    float *u, float *velocity, float *density, float *damp,
+    float *wavelet, size_t wavelet_size, size_t wavelet_count,
+    float *coeff_order2, float *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    float *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    float *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    float *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, float dz, float dx, float dy,
+    size_t saving_stride, float dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_3d_vd_f32
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_3d_vd_f32(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or
@@ -924,6 +1129,35 @@ double simwave_hip_forward_device_3d_vd_f64(
     size_t saving_stride, double dt, size_t begin_timestep,
     size_t end_timestep, size_t space_order, size_t num_snapshots,
     void *stream, unsigned flags);
+/* Allocation-free device entry (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as simwave_hip_forward_device_3d_vd_f64, flags 0
+ * or 2 only; the library's buffers come from a static 64 MiB device
+ * array per library and device (per-point arrays of double: 24 bytes a grid
+ * point, so at most about 2.7 M points), and larger grids are rejected
+ * with an error.  No hipMalloc, hipFree or allocating library call.
+ * Synchronous; calls that overlap on one device (several threads)
+ * race on the shared array. */
+double simwave_hip_forward_device_noalloc_3d_vd_f64(
+    double *u, double *velocity, double *density, double *damp,
+    double *wavelet, size_t wavelet_size, size_t wavelet_count,
+    double *coeff_order2, double *coeff_order1,
+    size_t *boundary_conditions,
+    size_t *src_points_interval, size_t src_points_interval_size,
+    double *src_points_values, size_t src_points_values_size,
+    size_t *src_points_values_offset,
+    size_t *rec_points_interval, size_t rec_points_interval_size,
+    double *rec_points_values, size_t rec_points_values_size,
+    size_t *rec_points_values_offset,
+    double *receivers, size_t num_sources, size_t num_receivers,
+    size_t nz, size_t nx, size_t ny, double dz, double dx, double dy,
+    size_t saving_stride, double dt, size_t begin_timestep,
+    size_t end_timestep, size_t space_order, size_t num_snapshots,
+    void *stream, unsigned flags);
+/* Bytes of the static array of simwave_hip_forward_device_noalloc_3d_vd_f64
+ * (host-only helper). */
+size_t simwave_hip_scratch_bytes_3d_vd_f64(void);
 /* Persistent plan (no counterpart in the reference, whose forward runs a
  * whole step range per call): the device entry split into create / run in
  * chunks / destroy, for callers that interleave shots (the serving loop) or

@@ -57,6 +57,15 @@
 
 #define SW_EXPORT extern "C" __attribute__((visibility("default")))
 
+// Device memory of the allocation-free device entry
+// (simwave_hip_forward_device_noalloc_<variant>): the per-point coefficient
+// arrays, damping mask words, parameter copy and source/receiver tables of
+// one run are carved from this array instead of allocated.  Each device
+// holds its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 using sw::Box;
@@ -1841,6 +1850,90 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
     return total;
 }
 
+// Address of this library's scratch array on the current device, looked up
+// once per device (with that device current) and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+
+void *scratch_region(hipStream_t st)
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (st) {
+        int sdev = -1;
+        hip_check(hipStreamGetDevice(st, &sdev), "hipStreamGetDevice");
+        if (sdev != dev)
+            fail("the stream belongs to device %d but device %d is current; make the "
+                 "stream's device current", sdev, dev);
+    }
+    if (dev < 0 || dev >= kMaxDevices)
+        fail("device %d is beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p)
+            fail("no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// One whole shot on `stream` like run_device, with every library buffer in
+// the static scratch array: no device allocation or free.  Returns after the
+// stream has drained, so that the next call may reuse the array; calls that
+// overlap on one device (from several threads) race on it.
+double run_device_noalloc(const Args &a, hipStream_t st, unsigned flags)
+{
+    if (flags & ~kFlagGeneric)
+        fail("flags 0x%x: the allocation-free entry takes only flag %u (generic "
+             "kernels); step timing and the receiver stream need allocations",
+             flags, kFlagGeneric);
+    const double t0 = now_s();
+    for (double &t : g_timings)
+        t = 0.0;
+    validate(a);
+    const std::vector<StepSlots> slots = plan_slots(a);
+    for (size_t idx = 0; idx < slots.size(); idx++) {
+        const StepSlots &s = slots[idx];
+        size_t m = std::max({s.prev, s.cur, s.next});
+        if (s.swap_after)
+            m = std::max({m, s.swap_a, s.swap_b});
+        if (m >= a.num_snapshots)
+            fail("slot %zu used by step %zu is outside the %zu snapshots", m,
+                 a.n0 + idx, a.num_snapshots);
+    }
+    // the per-point arrays (q, e and, for the tiled 3D kernel, C) decide
+    // whether the grid fits; the small tables follow them
+    const size_t npts = a.nz * a.nx * (SW_DIM == 3 ? a.ny : 1);
+    const size_t per_point = (SW_DIM == 3 ? 3 : 2) * sizeof(real_t);
+    if (npts > (kScratchBytes - (1u << 20)) / per_point)
+        fail("grid of %zu points needs %zu bytes of per-point arrays, more than the "
+             "%zu-byte static scratch holds (at most %zu points); use "
+             "simwave_hip_forward_device for this grid",
+             npts, npts * per_point, kScratchBytes, (kScratchBytes - (1u << 20)) / per_point);
+    void *base = scratch_region(st);
+    DeviceArena arena(st, base, kScratchBytes);
+    const Prepared pr = prepare(a, true, flags, arena, st);
+    real_t *u = a.u;
+    auto slot_ptr = [&](size_t sl) { return u + sl * (size_t)pr.npts; };
+    auto nop = [](size_t) {};
+    auto swap = [&](size_t x, size_t y) {
+        const long long n = pr.npts;
+        const int blocks = (int)std::min<long long>((n + 255) / 256, 16384);
+        hipLaunchKernelGGL(sw::k_swap, dim3(blocks), dim3(256), 0, st, u + x * n, u + y * n, n);
+    };
+    auto no_uploads = [](size_t) { return false; };
+    const double tl0 = now_s();
+    time_loop(a, pr, slots, 0, slots.size(), a.receivers, 0, flags, st, slot_ptr, nop, nop,
+              swap, no_uploads);
+    hip_check(hipStreamSynchronize(st), "time loop");
+    g_timings[2] = now_s() - tl0;
+    g_timings[0] = now_s() - t0;
+    return g_timings[0];
+}
+
 // ------------------------------------------------------------- adjoint
 // Reverse interior update of one step: the z-marching kernels (3D, 2D);
 // the whole-grid k_adj_update for radii they do not carry.
@@ -2333,6 +2426,22 @@ SW_EXPORT double SW_SUFFIX(simwave_hip_forward_device_)(SW_FORWARD_PARAMS, void 
     return guarded([&] {
         return run_device(fill_args(SW_FORWARD_ARGS), (hipStream_t)stream, flags);
     });
+}
+
+SW_EXPORT double SW_SUFFIX(simwave_hip_forward_device_noalloc_)(SW_FORWARD_PARAMS,
+                                                                void *stream,
+                                                                unsigned flags)
+{
+    return guarded([&] {
+        return run_device_noalloc(fill_args(SW_FORWARD_ARGS), (hipStream_t)stream, flags);
+    });
+}
+
+// Host-only helper (no device work): size in bytes of the static scratch
+// array of the allocation-free entry.
+SW_EXPORT size_t SW_SUFFIX(simwave_hip_scratch_bytes_)(void)
+{
+    return kScratchBytes;
 }
 
 SW_EXPORT void *SW_SUFFIX(simwave_hip_plan_create_)(SW_FORWARD_PARAMS, void *stream,
