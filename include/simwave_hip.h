@@ -14,7 +14,9 @@
  *   - simwave_hip_forward_device_<variant>: the same contract with device
  *     pointers for the arrays that live on the GPU across calls;
  *   - simwave_hip_plan_{create,run,destroy}_<variant>: the device entry
- *     split into chunks for serving loops;
+ *     split into chunks for serving loops, and
+ *     simwave_hip_plans_run_<variant>, which advances several plans in
+ *     batched launches;
  *   - simwave_hip_forward_device_noalloc_<variant>: the device entry with
  *     every library buffer in a static device array (no allocation);
  *   - simwave_hip_fixed_div_<variant>, simwave_hip_last_error,
@@ -146,6 +148,18 @@ void *simwave_hip_plan_create_2d_cd_f32(
 long long simwave_hip_plan_run_2d_cd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_2d_cd_f32) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_2d_cd_f32.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_2d_cd_f32(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_2d_cd_f32(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_2d_cd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -160,7 +174,9 @@ int simwave_hip_plan_destroy_2d_cd_f32(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -295,6 +311,18 @@ void *simwave_hip_plan_create_2d_cd_f64(
 long long simwave_hip_plan_run_2d_cd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_2d_cd_f64) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_2d_cd_f64.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_2d_cd_f64(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_2d_cd_f64(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_2d_cd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -309,7 +337,9 @@ int simwave_hip_plan_destroy_2d_cd_f64(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -444,6 +474,18 @@ void *simwave_hip_plan_create_2d_vd_f32(
 long long simwave_hip_plan_run_2d_vd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_2d_vd_f32) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_2d_vd_f32.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_2d_vd_f32(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_2d_vd_f32(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_2d_vd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -458,7 +500,9 @@ int simwave_hip_plan_destroy_2d_vd_f32(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -593,6 +637,18 @@ void *simwave_hip_plan_create_2d_vd_f64(
 long long simwave_hip_plan_run_2d_vd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_2d_vd_f64) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_2d_vd_f64.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_2d_vd_f64(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_2d_vd_f64(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_2d_vd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -607,7 +663,9 @@ int simwave_hip_plan_destroy_2d_vd_f64(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -742,6 +800,18 @@ void *simwave_hip_plan_create_3d_cd_f32(
 long long simwave_hip_plan_run_3d_cd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_3d_cd_f32) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_3d_cd_f32.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_3d_cd_f32(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_3d_cd_f32(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_3d_cd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -756,7 +826,9 @@ int simwave_hip_plan_destroy_3d_cd_f32(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -891,6 +963,18 @@ void *simwave_hip_plan_create_3d_cd_f64(
 long long simwave_hip_plan_run_3d_cd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_3d_cd_f64) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_3d_cd_f64.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_3d_cd_f64(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_3d_cd_f64(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_3d_cd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -905,7 +989,9 @@ int simwave_hip_plan_destroy_3d_cd_f64(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -1040,6 +1126,18 @@ void *simwave_hip_plan_create_3d_vd_f32(
 long long simwave_hip_plan_run_3d_vd_f32(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_3d_vd_f32) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_3d_vd_f32.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_3d_vd_f32(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_3d_vd_f32(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_3d_vd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -1054,7 +1152,9 @@ int simwave_hip_plan_destroy_3d_vd_f32(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */
@@ -1189,6 +1289,18 @@ void *simwave_hip_plan_create_3d_vd_f64(
 long long simwave_hip_plan_run_3d_vd_f64(void *plan, size_t steps, void *stream);
 /* Frees the plan once its enqueued work is done.  Returns 0, or -1. */
 int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
+/* Batched runs (no counterpart in the reference; for the serving loop):
+ * enqueues up to `steps` further steps of each of the `count` plans on
+ * `stream`, after each plan's earlier work.  Plans that can be batched
+ * (simwave_hip_plan_batchable_3d_vd_f64) advance together, radius by radius:
+ * one step kernel and one receiver kernel per time step for all of them
+ * (up to 48 per launch); the others run as simwave_hip_plan_run_3d_vd_f64.
+ * left[i] receives plan i's remaining steps.  Returns 0, or -1. */
+int simwave_hip_plans_run_3d_vd_f64(void **plans, int count, size_t steps,
+                            void *stream, long long *left);
+/* 1 when the plan can join batched runs (stride 0, fused z-marching step
+ * kernel, no source box in the halo, flags without 1 and 4), else 0. */
+int simwave_hip_plan_batchable_3d_vd_f64(void *plan);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_3d_vd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -1203,7 +1315,9 @@ int simwave_hip_plan_destroy_3d_vd_f64(void *plan);
  * rec_adjoint as observed data and drives the adjoint with the residual
  * (shot record - rec_adjoint) formed on the device after the forward (the
  * L2 misfit's gradient in one call); 4 runs the forward's receiver gather
- * on a second stream (large grids).
+ * on a second stream (large grids); 8 times every reverse update kernel
+ * with HIP events (simwave_hip_last_timings: [4] their summed seconds,
+ * [5] their count).
  * The shot starts from rest: u's three slots are zeroed first, whatever
  * they hold.  Also writes the shot record and leaves u as after a plain
  * forward.  Returns wall seconds, or -1. */

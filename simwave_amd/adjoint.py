@@ -29,6 +29,7 @@ __all__ = ["gradient", "misfit_gradient", "shot_misfit_gradient", "image", "Shot
 IMAGE = 1     # adjoint entry flag: cross-correlation image instead of gradient
 RESIDUAL = 2  # adjoint entry flag: rec_adjoint is observed data, the source the residual
 OVERLAP_RECEIVERS = 4   # forward receiver gather on a second stream (large grids)
+TIME_UPDATE = 8         # adjoint entry flag: HIP-event timing of every reverse update kernel
 
 
 def _check(solver):
@@ -58,7 +59,7 @@ def checkpoint_interval(dev, checkpoint_every=None):
 
 
 def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=False,
-         overlap_receivers=True, residual=False):
+         overlap_receivers=True, residual=False, time_update=False):
     import torch
     kw = dev.kwargs
     checkpoint_every = checkpoint_interval(dev, checkpoint_every)
@@ -87,7 +88,8 @@ def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=Fa
               ctypes.c_void_p(gw.data_ptr() if gw is not None else 0),
               ctypes.c_size_t(int(checkpoint_every)), ctypes.c_void_p(stream.cuda_stream),
               ctypes.c_uint((IMAGE if image else 0) | (RESIDUAL if residual else 0)
-                            | (OVERLAP_RECEIVERS if overlap_receivers else 0)))
+                            | (OVERLAP_RECEIVERS if overlap_receivers else 0)
+                            | (TIME_UPDATE if time_update else 0)))
     if secs < 0:
         raise RuntimeError("adjoint failed: {}".format(_last_error(lib)))
     return grad, gw
@@ -146,16 +148,20 @@ def misfit_gradient(solver, observed, checkpoint_every=None, velocity=None, devi
     return misfit, grad.cpu().numpy()
 
 
-def shot_misfit_gradient(dev, observed, checkpoint_every=None, stream=None):
+def shot_misfit_gradient(dev, observed, checkpoint_every=None, stream=None,
+                         time_update=False):
     """One call of the adjoint entry in residual mode on a device-resident
     shot: the forward, the residual d - observed on the device, the reverse
     sweep.  Returns (misfit 1/2 ||d - observed||^2 as a float, gradient
-    with respect to the extended velocity as a float64 device tensor)."""
+    with respect to the extended velocity as a float64 device tensor).
+    With time_update the reverse update kernels are timed with HIP events
+    (``dev.timings()[4:6]``: summed seconds, launches)."""
     import torch
     rec = dev.tensors["shot_record"]
     obs = torch.as_tensor(observed, device=dev.device).to(rec.dtype).contiguous()
     dev.reset()
-    grad, _ = _run(dev, obs, checkpoint_every, False, stream=stream, residual=True)
+    grad, _ = _run(dev, obs, checkpoint_every, False, stream=stream, residual=True,
+                   time_update=time_update)
     res = rec.double() - obs.double()
     return 0.5 * float((res * res).sum()), grad
 

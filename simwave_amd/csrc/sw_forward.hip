@@ -17,6 +17,7 @@
 #include "sw_kernels.hpp"
 #include "sw_step3d.hpp"
 #include "sw_adjoint.hpp"
+#include "sw_batch.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -117,6 +118,9 @@ constexpr unsigned kAdjImage = 1u;
 // adjoint entry: rec_adjoint holds observed data; the adjoint source is the
 // residual (shot record - observed), formed on the device after the forward
 constexpr unsigned kAdjResidual = 2u;
+// adjoint entry: time every reverse update kernel with HIP events
+// (simwave_hip_last_timings: [4] their summed seconds, [5] their count)
+constexpr unsigned kAdjTimeUpdate = 8u;
 // reverse steps whose gradient terms are summed in the working precision
 // before they are added to the float64 gradient
 constexpr size_t kGradFlush = 16;
@@ -915,21 +919,30 @@ bool zmarch_2d(int R)
            ((sizeof(real_t) == 4 || R <= 4) && !env_flag("SIMWAVE_HIP_UNTILED"));
 }
 
+// Rows per block of the 2D step kernels: about 2048 blocks, down to one
+// row each: a block's rows form one dependent load chain, so small grids run
+// fastest with the most blocks (517^2 so4: 7.7 us per step at 1 row against
+// 14.4 us at 9 rows; 1136 x 3136 so16: 57 us at 8 rows, within 3% of the
+// best)
+int zlen_2d(const Params &P, int R)
+{
+    const int Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    const int gx = (Ix + kTB - 1) / kTB;
+    int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
+    if (zlen <= 0) {
+        const int nzc = std::min(std::max(1, (2048 + gx - 1) / gx), std::max(1, Iz));
+        zlen = (Iz + nzc - 1) / nzc;
+    }
+    return zlen;
+}
+
 template <int R>
 void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *un,
                   int n, hipStream_t st)
 {
     const int Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
     const int gx = (Ix + kTB - 1) / kTB;
-    // about 2048 blocks, down to one row each: a block's rows form one
-    // dependent load chain, so small grids run fastest with the most blocks
-    // (517^2 so4: 7.7 us per step at 1 row against 14.4 us at 9 rows;
-    // 1136 x 3136 so16: 57 us at 8 rows, within 3% of the best)
-    int zlen = env_int("SIMWAVE_HIP_ZLEN", 0);
-    if (zlen <= 0) {
-        const int nzc = std::min(std::max(1, (2048 + gx - 1) / gx), std::max(1, Iz));
-        zlen = (Iz + nzc - 1) / nzc;
-    }
+    const int zlen = zlen_2d(P, R);
     const int gz = (Iz + zlen - 1) / zlen;
     // the z-marching kernel for float32 (0-27% faster than the LDS row
     // kernel on 2D grids of 304^2 to 1136 x 3136, orders 2-20, CD and VD:
@@ -1677,6 +1690,11 @@ struct Plan {
     int *nbase = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // batched runs (run_plans_batched): the shot's constants on the device
+    // and the blocks of its step and receiver grids; null when the plan
+    // runs only on its own
+    const sw::BatchShot *bshot = nullptr;
+    int bblocks = 0, rblocks = 0;
     Plan(const Args &args, hipStream_t s, unsigned f) : a(args), flags(f), st(s), arena(s) {}
     ~Plan()
     {
@@ -1758,6 +1776,159 @@ void capture_graph(Plan &p)
     hip_check(hipGraphInstantiate(&p.gexec, p.graph, nullptr, nullptr, 0), "graph instantiate");
 }
 
+size_t run_plan(Plan &p, size_t count, hipStream_t st);
+
+// A plan can join batched runs when its steps use the fused z-marching
+// kernel of a stride-0 run with no source in the halo and no per-step
+// timing or receiver stream.
+void setup_batch(Plan &p)
+{
+    const Prepared &pr = p.pr;
+    const Params &P = pr.P;
+    const int R = pr.r;
+    if (!pr.fused || pr.halo_sources || p.a.stride != 0 || !periodic_slots(p.slots) ||
+        (p.flags & (kFlagTimeSteps | kFlagOverlapReceivers)) || env_flag("SIMWAVE_HIP_NO_BATCH"))
+        return;
+    sw::BatchShot b{};
+#if SW_DIM == 3
+    if (!pr.zmarch)
+        return;
+    b.D = pr.lapdiv;
+    b.zlen = pr.zmarch_zlen;
+    b.gx = (P.ny - 2 * R + 63) / 64;
+    b.gy = (P.nx - 2 * R + sw::kZLines - 1) / sw::kZLines;
+    const long long blocks =
+        (long long)b.gx * b.gy * ((P.nz - 2 * R + b.zlen - 1) / b.zlen);
+#else
+    if (!zmarch_2d(R))
+        return;
+    b.zlen = zlen_2d(P, R);
+    b.gx = (P.nx - 2 * R + kTB - 1) / kTB;
+    b.gy = 1;
+    const long long blocks = (long long)b.gx * ((P.nz - 2 * R + b.zlen - 1) / b.zlen);
+#endif
+    if (blocks >= (1ll << 28))
+        return;
+    b.P = P;
+    b.P.nbase = nullptr;
+    b.u = p.a.u;
+    b.rec = p.a.receivers;
+    b.rbox = pr.rbox;
+    b.rw = pr.rw;
+    b.npts = pr.npts;
+    b.nrec = (int)p.a.nrec;
+    sw::BatchShot *d = p.arena.alloc<sw::BatchShot>(1);
+    hip_check(hipMemcpyAsync(d, &b, sizeof(b), hipMemcpyHostToDevice, p.st),
+              "copy batch constants");
+    p.bshot = d;
+    p.bblocks = (int)blocks;
+    p.rblocks = (int)((p.a.nrec + kRecPerBlock - 1) / kRecPerBlock);
+}
+
+template <int R>
+void launch_step_batch(const sw::BatchArgs &A, hipStream_t st)
+{
+#if SW_DIM == 3
+    hipLaunchKernelGGL(sw::k_step3d_z_batch<R>, dim3((unsigned)A.first[A.count]),
+                       dim3(64, sw::kZLines), 0, st, A);
+#else
+    hipLaunchKernelGGL(sw::k_step2d_z_batch<R>, dim3((unsigned)A.first[A.count]),
+                       dim3(sw::kZ2Block), 0, st, A);
+#endif
+}
+
+using BatchFn = void (*)(const sw::BatchArgs &, hipStream_t);
+constexpr BatchFn kStepBatch[] = {nullptr,
+                                  &launch_step_batch<1>, &launch_step_batch<2>,
+                                  &launch_step_batch<3>, &launch_step_batch<4>,
+                                  &launch_step_batch<5>, &launch_step_batch<6>,
+                                  &launch_step_batch<7>, &launch_step_batch<8>,
+                                  &launch_step_batch<9>, &launch_step_batch<10>};
+
+// Enqueues up to `steps` further steps of every plan on `st`, after each
+// plan's earlier work.  Plans that can be batched (setup_batch) advance
+// together, radius by radius, up to kMaxBatch per launch: one step kernel
+// and one receiver kernel per time step for all of them, in segments over
+// which the set of unfinished plans is fixed.  The others run on their own
+// (run_plan).  left[i]: steps plan i has left.
+void run_plans_batched(Plan *const *plans, int count, size_t steps, hipStream_t st,
+                       long long *left)
+{
+    for (int i = 0; i < count; i++) {
+        if (!plans[i])
+            fail("NULL plan %d", i);
+        for (int j = 0; j < i; j++)
+            if (plans[j] == plans[i])
+                fail("plan %d listed twice", i);
+    }
+    std::vector<Plan *> group[sw::kMaxR + 1];
+    for (int i = 0; i < count; i++) {
+        Plan *p = plans[i];
+        if (p->bshot) {
+            hip_check(hipStreamWaitEvent(st, p->last, 0), "hipStreamWaitEvent");
+            group[p->pr.r].push_back(p);
+        } else {
+            run_plan(*p, steps, st);
+        }
+    }
+    for (int R = 1; R <= sw::kMaxR; R++) {
+        std::vector<Plan *> &g = group[R];
+        std::vector<size_t> want(g.size());
+        for (size_t i = 0; i < g.size(); i++)
+            want[i] = std::min(g[i]->slots.size(), g[i]->done + steps);
+        for (;;) {
+            std::vector<Plan *> act;
+            size_t seg = std::numeric_limits<size_t>::max();
+            for (size_t i = 0; i < g.size(); i++)
+                if (g[i]->done < want[i]) {
+                    act.push_back(g[i]);
+                    seg = std::min(seg, want[i] - g[i]->done);
+                }
+            if (act.empty())
+                break;
+            for (size_t c0 = 0; c0 < act.size(); c0 += sw::kMaxBatch) {
+                const int cnt = (int)std::min<size_t>(sw::kMaxBatch, act.size() - c0);
+                sw::BatchArgs S{}, Rc{};
+                S.count = cnt;
+                S.first[0] = 0;
+                int rc = 0;
+                for (int i = 0; i < cnt; i++) {
+                    const Plan *p = act[c0 + i];
+                    S.shot[i] = p->bshot;
+                    S.n[i] = (int)(p->a.n0 + p->done);
+                    S.first[i + 1] = S.first[i] + p->bblocks;
+                    if (p->rblocks) {
+                        Rc.shot[rc] = p->bshot;
+                        Rc.n[rc] = S.n[i];
+                        Rc.first[rc + 1] = Rc.first[rc] + p->rblocks;
+                        rc++;
+                    }
+                }
+                Rc.count = rc;
+                for (size_t k = 0; k < seg; k++) {
+                    kStepBatch[R](S, st);
+                    if (rc)
+                        hipLaunchKernelGGL((sw::k_receivers_batch<kRecPerBlock, kRecChunk>),
+                                           dim3((unsigned)Rc.first[rc]), dim3(256), 0, st, Rc);
+                    hip_check(hipGetLastError(), "batched step");
+                    for (int i = 0; i < cnt; i++)
+                        S.n[i]++;
+                    for (int i = 0; i < rc; i++)
+                        Rc.n[i]++;
+                }
+            }
+            for (Plan *p : act)
+                p->done += seg;
+        }
+    }
+    for (int i = 0; i < count; i++) {
+        Plan *p = plans[i];
+        if (p->bshot)
+            hip_check(hipEventRecord(p->last, st), "hipEventRecord");
+        left[i] = (long long)(p->slots.size() - p->done);
+    }
+}
+
 Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
 {
     validate(a);
@@ -1779,6 +1950,7 @@ Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
     if ((flags & kFlagGraph) && !(flags & (kFlagOverlapReceivers | kFlagTimeSteps)) &&
         p->slots.size() >= kGraphSteps && periodic_slots(p->slots))
         capture_graph(*p);
+    setup_batch(*p);
     hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventRecord(p->last, st), "hipEventRecord");
     return p.release();
@@ -2108,9 +2280,9 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
         fail("NULL gradient array");
     if (a.nrec && !rec_adj)
         fail("NULL shot-record adjoint");
-    if (flags & ~(kAdjImage | kAdjResidual | kFlagOverlapReceivers))
-        fail("unknown adjoint flags 0x%x",
-             flags & ~(kAdjImage | kAdjResidual | kFlagOverlapReceivers));
+    const unsigned known = kAdjImage | kAdjResidual | kFlagOverlapReceivers | kAdjTimeUpdate;
+    if (flags & ~known)
+        fail("unknown adjoint flags 0x%x", flags & ~known);
     const bool image = flags & kAdjImage;
     const int r = (int)(a.space_order / 2);
     const size_t dims[3] = {a.nz, a.nx, SW_DIM == 3 ? a.ny : (size_t)(3 * r + 1)};
@@ -2234,6 +2406,23 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     }
     const double t_fwd = now_s();
 
+    // event pairs around every reverse update kernel (kAdjTimeUpdate)
+    struct UpdateTimer {
+        std::vector<hipEvent_t> ev;
+        size_t used = 0;
+        ~UpdateTimer()
+        {
+            for (hipEvent_t e : ev)
+                if (e)
+                    (void)hipEventDestroy(e);
+        }
+    } utime;
+    if (flags & kAdjTimeUpdate) {
+        utime.ev.assign(2 * T, nullptr);
+        for (hipEvent_t &e : utime.ev)
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+    }
+
     // backward over the checkpoint segments
     Args seg_args = a;
     seg_args.nrec = 0;
@@ -2286,8 +2475,15 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
             if (a.nsrc && gw)
                 hipLaunchKernelGGL(sw::k_adj_sources, dim3((unsigned)a.nsrc), dim3(256), 0, st,
                                    P, ub, gw + (n - 1) * a.nsrc);
+            const bool timed = utime.used + 2 <= utime.ev.size();
+            if (timed)
+                hip_check(hipEventRecord(utime.ev[utime.used], st), "hipEventRecord");
             adj_update(P, st, a.vel, ub, W, ub1, ub2, seg + (n + 1 - m) * npts,
                        seg + (n - m) * npts, seg + (n - 1 - m) * npts, gacc, image ? 1 : 0);
+            if (timed) {
+                hip_check(hipEventRecord(utime.ev[utime.used + 1], st), "hipEventRecord");
+                utime.used += 2;
+            }
             if (++reverse_steps % kGradFlush == 0)
                 flush_grad();
             hip_check(hipGetLastError(), "adjoint step");
@@ -2297,6 +2493,17 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
     }
     flush_grad();
     hip_check(hipStreamSynchronize(st), "adjoint");
+    if (utime.used) {
+        double sec = 0.0;
+        for (size_t i = 0; i < utime.used; i += 2) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, utime.ev[i], utime.ev[i + 1]),
+                      "hipEventElapsedTime");
+            sec += ms * 1e-3;
+        }
+        g_timings[4] = sec;
+        g_timings[5] = (double)(utime.used / 2);
+    }
     g_timings[1] = t_fwd - t0;
     g_timings[2] = now_s() - t_fwd;
     const double total = now_s() - t0;
@@ -2467,6 +2674,24 @@ SW_EXPORT long long SW_SUFFIX(simwave_hip_plan_run_)(void *plan, size_t steps, v
         return 0.0;
     });
     return r < 0 ? -1 : left;
+}
+
+SW_EXPORT int SW_SUFFIX(simwave_hip_plans_run_)(void **plans, int count, size_t steps,
+                                                void *stream, long long *left)
+{
+    const double r = guarded([&] {
+        if (count < 0 || (count && (!plans || !left)))
+            fail("bad plan list");
+        run_plans_batched(reinterpret_cast<Plan *const *>(plans), count, steps,
+                          (hipStream_t)stream, left);
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int SW_SUFFIX(simwave_hip_plan_batchable_)(void *plan)
+{
+    return plan && static_cast<Plan *>(plan)->bshot ? 1 : 0;
 }
 
 SW_EXPORT int SW_SUFFIX(simwave_hip_plan_destroy_)(void *plan)

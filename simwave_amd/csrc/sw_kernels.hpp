@@ -479,16 +479,19 @@ k_step3d(Params P, const real_t *__restrict__ uc, const real_t *up,
 // slower on deep so8 cubes (176^3-352^3: 1-2.5%)
 constexpr int kZLines = 8;
 
+// One block of the step: block (bx, by, bz) of the (y, x, z-chunk) grid.
 template <int R>
-__global__ void __launch_bounds__(64 * kZLines)
-k_step3d_z(Params P, LapDiv D, const real_t *__restrict__ uc, const real_t *__restrict__ up,
-           real_t *__restrict__ un, int n, int zlen)
+__device__ __forceinline__ void step3d_z_block(const Params &P, const LapDiv &D,
+                                               const real_t *__restrict__ uc,
+                                               const real_t *__restrict__ up,
+                                               real_t *__restrict__ un, int n, int zlen,
+                                               int bx, int by, int bz)
 {
-    const int k = R + blockIdx.x * 64 + threadIdx.x;
-    const int j = R + blockIdx.y * kZLines + threadIdx.y;
+    const int k = R + bx * 64 + threadIdx.x;
+    const int j = R + by * kZLines + threadIdx.y;
     if (k >= P.ny - R || j >= P.nx - R)
         return;
-    const int z0 = R + blockIdx.z * zlen;
+    const int z0 = R + bz * zlen;
     const int z1 = min(z0 + zlen, P.nz - R);
     const long long sz = P.sz, sx = P.sx;
     const long long col = (long long)j * sx + k;
@@ -592,6 +595,14 @@ k_step3d_z(Params P, LapDiv D, const real_t *__restrict__ uc, const real_t *__re
         store_with_bc(P, un, p, i, j, k, v);
         cur = nxt;
     }
+}
+
+template <int R>
+__global__ void __launch_bounds__(64 * kZLines)
+k_step3d_z(Params P, LapDiv D, const real_t *__restrict__ uc, const real_t *__restrict__ up,
+           real_t *__restrict__ un, int n, int zlen)
+{
+    step3d_z_block<R>(P, D, uc, up, un, n, zlen, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 #endif
 
@@ -712,15 +723,17 @@ k_step2d(Params P, const real_t *__restrict__ uc, const real_t *up,
 // operations in the same order as k_step2d.
 constexpr int kZ2Block = 256;
 
+// One block of the step: block (bx, bz) of the (x, z-chunk) grid.
 template <int R>
-__global__ void __launch_bounds__(kZ2Block)
-k_step2d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ up,
-           real_t *__restrict__ un, int n, int zlen)
+__device__ __forceinline__ void step2d_z_block(const Params &P, const real_t *__restrict__ uc,
+                                               const real_t *__restrict__ up,
+                                               real_t *__restrict__ un, int n, int zlen,
+                                               int bx, int bz)
 {
-    const int j = R + blockIdx.x * kZ2Block + threadIdx.x;
+    const int j = R + bx * kZ2Block + threadIdx.x;
     if (j >= P.nx - R)
         return;
-    const int z0 = R + blockIdx.y * zlen;
+    const int z0 = R + bz * zlen;
     const int z1 = min(z0 + zlen, P.nz - R);
     const long long sz = P.sz;
     real_t qz[2 * R + 1];
@@ -813,6 +826,14 @@ k_step2d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ u
         store_with_bc(P, un, p, i, j, 0, v);
         cur = nxt;
     }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kZ2Block)
+k_step2d_z(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ up,
+           real_t *__restrict__ un, int n, int zlen)
+{
+    step2d_z_block<R>(P, uc, up, un, n, zlen, blockIdx.x, blockIdx.y);
 }
 #endif
 
@@ -1054,19 +1075,20 @@ k_receivers(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbo
 // conflicts.  Every product and every partial sum is
 // rounded exactly as in the reference's sequential loop (wave.c:533-565); only
 // the loads and multiplies run in parallel.
+// Receivers bx*RPB .. bx*RPB+RPB-1 of one shot.
 template <int RPB, int CH>
-__global__ void __launch_bounds__(256)
-k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbox,
-                 const real_t *__restrict__ rw, int nrec, real_t *__restrict__ out)
+__device__ __forceinline__ void receivers_coop_block(const Params &P,
+                                                     const real_t *__restrict__ uc,
+                                                     const Box *__restrict__ rbox,
+                                                     const real_t *__restrict__ rw, int nrec,
+                                                     real_t *__restrict__ out, int bx)
 {
-    if (P.nbase)
-        out += (long long)(*P.nbase) * nrec;
     __shared__ real_t prod[CH][RPB + 1];
     __shared__ Box boxes[RPB];
     __shared__ int counts[RPB];
     __shared__ int maxcount;
     const int tid = threadIdx.x;
-    const int r0 = blockIdx.x * RPB;
+    const int r0 = bx * RPB;
     if (tid == 0)
         maxcount = 0;
     __syncthreads();
@@ -1143,6 +1165,16 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
     }
     if (tid < RPB && r0 + tid < nrec)
         out[r0 + tid] = acc;
+}
+
+template <int RPB, int CH>
+__global__ void __launch_bounds__(256)
+k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict__ rbox,
+                 const real_t *__restrict__ rw, int nrec, real_t *__restrict__ out)
+{
+    if (P.nbase)
+        out += (long long)(*P.nbase) * nrec;
+    receivers_coop_block<RPB, CH>(P, uc, rbox, rw, nrec, out, blockIdx.x);
 }
 
 // Zero-fills a word array (a kernel rather than a memset, so it behaves the

@@ -112,6 +112,14 @@ class DeviceForward:
         self._plan_destroy = getattr(lib, "simwave_hip_plan_destroy_" + self.variant)
         self._plan_destroy.restype = ctypes.c_int
         self._plan_destroy.argtypes = [ctypes.c_void_p]
+        self._plans_run = getattr(lib, "simwave_hip_plans_run_" + self.variant)
+        self._plans_run.restype = ctypes.c_int
+        self._plans_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                    ctypes.c_size_t, ctypes.c_void_p,
+                                    ctypes.POINTER(ctypes.c_longlong)]
+        self._plan_batchable = getattr(lib, "simwave_hip_plan_batchable_" + self.variant)
+        self._plan_batchable.restype = ctypes.c_int
+        self._plan_batchable.argtypes = [ctypes.c_void_p]
         self._plan = None
         self._argtypes = argtypes
         # the uploads and gathers above are queued on the current stream;
@@ -177,6 +185,36 @@ class DeviceForward:
             from .middleware import _last_error
             raise RuntimeError("plan run failed: {}".format(_last_error(self._lib)))
         return left
+
+    @property
+    def batchable(self):
+        """Whether the planned shot can advance in batched launches with
+        other shots of its library (``run_batched``): a stride-0 run on the
+        fused z-marching kernels with no source in the halo."""
+        return bool(self._plan) and self._plan_batchable(ctypes.c_void_p(self._plan)) == 1
+
+    @staticmethod
+    def run_batched(shots, steps, stream):
+        """Enqueues up to `steps` further steps of every planned shot in
+        `shots` (all of one library) on `stream`, batched where they can be
+        (``simwave_hip_plans_run_<variant>``); returns the steps each has
+        left."""
+        if not shots:
+            return []
+        lib = shots[0]
+        for s in shots:
+            if s._plan is None:
+                raise RuntimeError("no plan (call plan() first)")
+            if s._lib is not lib._lib or s.variant != lib.variant:
+                raise ValueError("batched shots must share one library")
+        n = len(shots)
+        plans = (ctypes.c_void_p * n)(*[s._plan for s in shots])
+        left = (ctypes.c_longlong * n)()
+        if lib._plans_run(plans, n, ctypes.c_size_t(int(steps)),
+                          ctypes.c_void_p(stream.cuda_stream), left) != 0:
+            from .middleware import _last_error
+            raise RuntimeError("batched run failed: {}".format(_last_error(lib._lib)))
+        return list(left)
 
     def release_plan(self):
         """Frees the plan's device memory once its enqueued work is done."""

@@ -42,14 +42,22 @@ def modelled_shots(setups, device="cuda"):
     return shots
 
 
-def misfit_gradient(shots, observed, checkpoint_every=20):
+def misfit_gradient(shots, observed, checkpoint_every=20, stats=None):
     """(J, dJ/dv): the summed misfit (float) and gradient (float64 device
-    tensor of the extended shape) of `shots` against `observed`."""
+    tensor of the extended shape) of `shots` against `observed`.  With a
+    `stats` dictionary the reverse update kernels are timed (HIP events)
+    and their summed seconds and launches added to stats["update_s"] and
+    stats["update_launches"]."""
     import torch
     total = None
     misfit = 0.0
     for dev, obs in zip(shots, observed):
-        m, g = adjoint.shot_misfit_gradient(dev, obs, checkpoint_every)
+        m, g = adjoint.shot_misfit_gradient(dev, obs, checkpoint_every,
+                                            time_update=stats is not None)
+        if stats is not None:
+            t = dev.timings()
+            stats["update_s"] = stats.get("update_s", 0.0) + t[4]
+            stats["update_launches"] = stats.get("update_launches", 0) + int(t[5])
         misfit += m
         total = g if total is None else total.add_(g)
     torch.cuda.synchronize()

@@ -6,19 +6,26 @@ while others run.  The server
 * admits requests in arrival order against an HBM budget (every admitted
   shot keeps its wavefield slots, models, shot record and the library's
   per-point arrays resident) and a cap on concurrent shots;
-* runs every admitted shot on its own HIP stream, driven by its own worker
-  thread through the library's persistent plan (``simwave_hip_plan_*``),
-  in chunks of time steps, so the kernels of different shots overlap on the
-  device and their host-side launch work runs in parallel (the ctypes calls
-  release the GIL); a stride-0 shot's chunks replay a hipGraph its plan
-  captured once;
+* prepares every admitted shot on its own worker thread and HIP stream
+  (device-resident inputs, the library's persistent plan,
+  ``simwave_hip_plan_*``);
+* advances the shots that can be batched (stride 0 on the fused z-marching
+  kernels: ``DeviceForward.batchable``) together: one runner thread per
+  variant library calls ``simwave_hip_plans_run_<variant>`` on its own
+  stream, which enqueues for every time step one step kernel and one
+  receiver kernel covering all running shots of a radius, so shots join
+  the running batch as they are admitted and leave it when their last step
+  is enqueued;
+* runs every other shot on its worker's stream in chunks of time steps (a
+  stride-0 shot's chunks replay a hipGraph its plan captured once), so the
+  kernels of different shots and batches overlap on the device;
 * frees a finished shot's memory at once, which admits the next waiting
   request, and resolves the request's future with ``(u_full, shot_record)``
   exactly as ``Solver.forward()`` returns them;
 * records per-request latency (arrival to result on the host).
 
-Every shot runs the same kernels as when it runs alone, with the same
-inputs, so its result is bit-identical to ``Solver.forward()``.
+A batched shot runs the same per-point operations on the same inputs as
+alone, so every result is bit-identical to ``Solver.forward()``.
 """
 import collections
 import threading
@@ -91,6 +98,7 @@ class ServeStats:
             "prep_plan_mean_s": float(np.mean([r.get("plan", np.nan) for r in self.records])),
             "device_mean_s": float(np.mean([r.get("device", np.nan) for r in self.records])),
             "fetch_mean_s": float(np.mean([r.get("fetch", np.nan) for r in self.records])),
+            "batched": int(sum(1 for r in self.records if r.get("batched"))),
         }
 
 
@@ -118,6 +126,8 @@ class ShotServer:
         every full chunk of a stride-0 shot replay the plan's graph).
     plan_flags : int, optional
         Flags of the plans (default: FLAG_GRAPH, captured-graph replay).
+    batch : bool
+        Advance batchable shots together (one runner per variant library).
     device : str
     shot_factory : callable, optional
         ``solver -> shot`` with the ``DeviceForward`` interface (``steps``,
@@ -126,7 +136,7 @@ class ShotServer:
     """
 
     def __init__(self, hbm_budget=200e9, max_active=16, chunk_steps=48, device="cuda",
-                 shot_factory=None, stream_factory=None, plan_flags=None):
+                 shot_factory=None, stream_factory=None, plan_flags=None, batch=True):
         self.hbm_budget = float(hbm_budget)
         self.max_active = int(max_active)
         self.chunk_steps = int(chunk_steps)
@@ -144,6 +154,8 @@ class ShotServer:
         self._closed = False
         self._workers = []
         self.admissions = []          # requests (as submitted) in admission order
+        self.batch = bool(batch)
+        self._runners = {}            # variant -> _BatchRunner
         self.stats = ServeStats()
         self._sched = threading.Thread(target=self._schedule, name="shot-scheduler",
                                        daemon=True)
@@ -169,6 +181,10 @@ class ShotServer:
             self._sched.join()
             for t in list(self._workers):
                 t.join()
+        with self._cv:
+            runners = list(self._runners.values())
+        for r in runners:
+            r.close(wait)
 
     def __enter__(self):
         return self
@@ -232,8 +248,9 @@ class ShotServer:
             stream = self._stream_factory()
             try:
                 import torch
-                ctx = torch.cuda.stream(stream)
-            except Exception:                  # test doubles: no torch stream
+                ctx = (torch.cuda.stream(stream) if isinstance(stream, torch.cuda.Stream)
+                       else _NullContext())     # test doubles: no torch stream
+            except ImportError:
                 ctx = _NullContext()
             t0 = time.perf_counter()
             with ctx:
@@ -241,9 +258,16 @@ class ShotServer:
             tb = time.perf_counter()
             shot.plan(stream, flags=self.plan_flags)
             t1 = time.perf_counter()
-            left = shot.steps
-            while left > 0:
-                left = shot.run_steps(self.chunk_steps, stream)
+            batched = self.batch and getattr(shot, "batchable", False)
+            if batched:
+                # the variant's runner enqueues the steps; this stream then
+                # waits for the plan's last work before the results
+                self._runner(shot).run(shot)
+                shot.run_steps(0, stream)
+            else:
+                left = shot.steps
+                while left > 0:
+                    left = shot.run_steps(self.chunk_steps, stream)
             stream.synchronize()
             t2 = time.perf_counter()
             result = shot.results(stream)
@@ -252,7 +276,7 @@ class ShotServer:
             self.stats.add(arrival=req.arrival, admitted=req.admitted, done=done,
                            updates=shot.updates_per_shot, active_at_admission=active_now,
                            prep=t1 - t0, build=tb - t0, plan=t1 - tb, device=t2 - t1,
-                           fetch=done - t2)
+                           fetch=done - t2, batched=batched)
             req.future.set_result(result)
         except Exception as exc:
             if shot is not None:
@@ -267,6 +291,81 @@ class ShotServer:
                 self._used -= need
                 self._active -= 1
                 self._cv.notify_all()
+
+
+    def _runner(self, shot):
+        with self._cv:
+            r = self._runners.get(shot.variant)
+            if r is None:
+                r = _BatchRunner(self._stream_factory(), self.chunk_steps,
+                                 type(shot).run_batched)
+                self._runners[shot.variant] = r
+            return r
+
+
+class _BatchRunner:
+    """Advances the batchable shots of one variant library on one stream:
+    each round enqueues up to `chunk` steps of every running shot in one
+    batched call; a shot whose last step is enqueued leaves the set and its
+    waiter is released (the device work is ordered before its results by
+    the plan's event).  At most two rounds are in flight on the device."""
+
+    def __init__(self, stream, chunk, run_batched):
+        self.stream = stream
+        self.chunk = chunk
+        self._run_batched = run_batched
+        self._cv = threading.Condition()
+        self._shots = {}              # shot -> [threading.Event, exception]
+        self._closed = False
+        self._thread = threading.Thread(target=self._loop, name="batch-runner", daemon=True)
+        self._thread.start()
+
+    def run(self, shot):
+        """Adds `shot` to the running set; returns once its last step is
+        enqueued (raises what the batched call raised)."""
+        slot = [threading.Event(), None]
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("runner is closed")
+            self._shots[shot] = slot
+            self._cv.notify_all()
+        slot[0].wait()
+        if slot[1] is not None:
+            raise slot[1]
+
+    def close(self, wait=True):
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        if wait:
+            self._thread.join()
+
+    def _loop(self):
+        inflight = collections.deque()
+        record = getattr(self.stream, "record_event", None)   # absent in test doubles
+        while True:
+            with self._cv:
+                while not self._shots and not self._closed:
+                    self._cv.wait()
+                if not self._shots:
+                    return
+                items = list(self._shots.items())
+            shots = [s for s, _ in items]
+            try:
+                left = self._run_batched(shots, self.chunk, self.stream)
+                err = None
+            except Exception as exc:      # every shot of the round fails
+                left, err = [0] * len(shots), exc
+            if record is not None:
+                inflight.append(record())
+                if len(inflight) > 2:
+                    inflight.popleft().synchronize()
+            with self._cv:
+                for (shot, slot), n in zip(items, left):
+                    if n <= 0:
+                        del self._shots[shot]
+                        slot[1] = err
+                        slot[0].set()
 
 
 class _NullContext:

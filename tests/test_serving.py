@@ -124,6 +124,39 @@ def test_submit_after_close_raises():
         srv.submit(FakeRequest(0, 1))
 
 
+class FakeBatchShot(FakeShot):
+    """A shot the server advances through its batch runner."""
+    batchable = True
+    variant = "fake"
+    rounds = []
+
+    @staticmethod
+    def run_batched(shots, steps, stream):
+        FakeBatchShot.rounds.append(len(shots))
+        time.sleep(0.001)
+        out = []
+        for s in shots:
+            s.left = max(0, s.left - steps)
+            out.append(s.left)
+        return out
+
+
+def test_batchable_shots_advance_together():
+    mon = Monitor()
+    FakeBatchShot.rounds = []
+    reqs = [FakeRequest(i, nbytes=1, steps=10 + 7 * i) for i in range(12)]
+    with ShotServer(shot_factory=lambda req: FakeBatchShot(req, mon),
+                    stream_factory=FakeStream, hbm_budget=100, max_active=12,
+                    chunk_steps=5) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        for r, f in zip(reqs, futs):
+            assert f.result(timeout=30) == (("u", r.ident), ("rec", r.ident))
+    assert max(FakeBatchShot.rounds) > 1          # several shots per batched call
+    s = srv.stats.summary()
+    assert s["batched"] == len(reqs)
+    assert mon.used == 0 and mon.active == 0
+
+
 def test_percentile_matches_numpy_linear():
     vals = [5.0, 1.0, 3.0, 2.0, 4.0]
     assert percentile(vals, 50) == 3.0
