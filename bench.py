@@ -645,8 +645,12 @@ def main():
         copy_calibration()
         return
     if args.cpu_only:
+        info = host_info()      # before the OpenMP runtime binds this thread
         kw, variant, what = cpu_workload(args.workload, _dtype(args), args.seed)
-        print(json.dumps(time_cpu(kw, variant, args.cpu_seconds, what)), flush=True)
+        out = time_cpu(kw, variant, args.cpu_seconds, what)
+        if out:
+            out.update(info)
+        print(json.dumps(out), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -420,8 +420,9 @@ double simwave_hip_forward_device_2d_vd_f32(
  * Arguments and results as simwave_hip_forward_device_2d_vd_f32, flags 0
  * or 2 only; the library's buffers come from a static 64 MiB device
  * array per library and device (per-point arrays of float: 8 bytes a grid
- * point, so at most about 8.3 M points), and larger grids are rejected
- * with an error.  No hipMalloc, hipFree or allocating library call.
+ * point, so at most about 8.3 M points; from space order 12 on, the density
+ * sums add 8 bytes a point: about 4.1 M points), and larger grids are
+ * rejected with an error.  No hipMalloc, hipFree or allocating library call.
  * Synchronous; calls that overlap on one device (several threads)
  * race on the shared array. */
 double simwave_hip_forward_device_noalloc_2d_vd_f32(
@@ -583,8 +584,9 @@ double simwave_hip_forward_device_2d_vd_f64(
  * Arguments and results as simwave_hip_forward_device_2d_vd_f64, flags 0
  * or 2 only; the library's buffers come from a static 64 MiB device
  * array per library and device (per-point arrays of double: 16 bytes a grid
- * point, so at most about 4.1 M points), and larger grids are rejected
- * with an error.  No hipMalloc, hipFree or allocating library call.
+ * point, so at most about 4.1 M points; from space order 12 on, the density
+ * sums add 16 bytes a point: about 2.0 M points), and larger grids are
+ * rejected with an error.  No hipMalloc, hipFree or allocating library call.
  * Synchronous; calls that overlap on one device (several threads)
  * race on the shared array. */
 double simwave_hip_forward_device_noalloc_2d_vd_f64(
@@ -1072,8 +1074,9 @@ double simwave_hip_forward_device_3d_vd_f32(
  * Arguments and results as simwave_hip_forward_device_3d_vd_f32, flags 0
  * or 2 only; the library's buffers come from a static 64 MiB device
  * array per library and device (per-point arrays of float: 12 bytes a grid
- * point, so at most about 5.5 M points), and larger grids are rejected
- * with an error.  No hipMalloc, hipFree or allocating library call.
+ * point, so at most about 5.5 M points; from space order 12 on, the density
+ * sums add 12 bytes a point: about 2.7 M points), and larger grids are
+ * rejected with an error.  No hipMalloc, hipFree or allocating library call.
  * Synchronous; calls that overlap on one device (several threads)
  * race on the shared array. */
 double simwave_hip_forward_device_noalloc_3d_vd_f32(
@@ -1235,8 +1238,9 @@ double simwave_hip_forward_device_3d_vd_f64(
  * Arguments and results as simwave_hip_forward_device_3d_vd_f64, flags 0
  * or 2 only; the library's buffers come from a static 64 MiB device
  * array per library and device (per-point arrays of double: 24 bytes a grid
- * point, so at most about 2.7 M points), and larger grids are rejected
- * with an error.  No hipMalloc, hipFree or allocating library call.
+ * point, so at most about 2.7 M points; from space order 12 on, the density
+ * sums add 24 bytes a point: about 1.3 M points), and larger grids are
+ * rejected with an error.  No hipMalloc, hipFree or allocating library call.
  * Synchronous; calls that overlap on one device (several threads)
  * race on the shared array. */
 double simwave_hip_forward_device_noalloc_3d_vd_f64(

@@ -373,6 +373,8 @@ struct Prepared {
 void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st);
 bool prefer_zmarch(const Prepared &pr);
 void set_lap_div(Prepared &pr);
+#else
+bool zmarch_2d(int R);
 #endif
 
 bool env_flag(const char *name)
@@ -420,6 +422,42 @@ void validate(const Args &a)
     if (a.nsrc && !a.wavelet)
         fail("NULL wavelet array");
 }
+
+#if SW_VD
+template <int R>
+void rho_diff_launch(const Params &P, real_t *dz, real_t *dm, real_t *dc, long long cnt,
+                     hipStream_t st)
+{
+    const unsigned blocks = (unsigned)std::min<long long>((cnt + 255) / 256, 16384);
+    hipLaunchKernelGGL(sw::k_rho_diff<R>, dim3(blocks), dim3(256), 0, st, P, dz, dm, dc);
+}
+
+// The density derivative sums the z-marching kernels read from radius
+// sw::kVdPre on (k_rho_diff), once per run.
+void rho_diff(Prepared &pr, DeviceArena &arena, hipStream_t st)
+{
+    Params &P = pr.P;
+    const int r = pr.r;
+    real_t *dz = arena.alloc<real_t>(pr.npts);
+    real_t *dm = SW_DIM == 3 ? arena.alloc<real_t>(pr.npts) : nullptr;
+    real_t *dc = arena.alloc<real_t>(pr.npts);
+    long long cnt = (long long)(P.nz - 2 * r) * (P.nx - 2 * r);
+    if (SW_DIM == 3)
+        cnt *= P.ny - 2 * r;
+    switch (r) {
+    case 6: rho_diff_launch<6>(P, dz, dm, dc, cnt, st); break;
+    case 7: rho_diff_launch<7>(P, dz, dm, dc, cnt, st); break;
+    case 8: rho_diff_launch<8>(P, dz, dm, dc, cnt, st); break;
+    case 9: rho_diff_launch<9>(P, dz, dm, dc, cnt, st); break;
+    case 10: rho_diff_launch<10>(P, dz, dm, dc, cnt, st); break;
+    default: fail("density sums for radius %d", r);
+    }
+    hip_check(hipGetLastError(), "k_rho_diff launch");
+    P.drho[0] = dz;
+    P.drho[1] = dm;
+    P.drho[2] = dc;
+}
+#endif
 
 // Uploads the run constants (the arguments passed validate()).
 // `dev_models` tells whether vel/rho/damp are device pointers (device
@@ -579,6 +617,15 @@ Prepared prepare(const Args &a, bool dev_models, unsigned flags,
         pr.zmarch_zlen = std::max(1, std::min(zl, 256));
         set_lap_div(pr);
     }
+#endif
+#if SW_VD
+#if SW_DIM == 3
+    const bool zm = pr.zmarch;
+#else
+    const bool zm = fused && zmarch_2d(r);
+#endif
+    if (zm && r >= sw::kVdPre)
+        rho_diff(pr, arena, st);
 #endif
     return pr;
 }
@@ -2079,7 +2126,10 @@ double run_device_noalloc(const Args &a, hipStream_t st, unsigned flags)
     // the per-point arrays (q, e and, for the tiled 3D kernel, C) decide
     // whether the grid fits; the small tables follow them
     const size_t npts = a.nz * a.nx * (SW_DIM == 3 ? a.ny : 1);
-    const size_t per_point = (SW_DIM == 3 ? 3 : 2) * sizeof(real_t);
+    // variable density from radius sw::kVdPre on adds the density sums
+    const size_t per_point =
+        ((SW_DIM == 3 ? 3 : 2) + (SW_VD && a.space_order / 2 >= (size_t)sw::kVdPre ? SW_DIM : 0)) *
+        sizeof(real_t);
     if (npts > (kScratchBytes - (1u << 20)) / per_point)
         fail("grid of %zu points needs %zu bytes of per-point arrays, more than the "
              "%zu-byte static scratch holds (at most %zu points); use "

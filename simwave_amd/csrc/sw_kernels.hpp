@@ -67,7 +67,16 @@ struct Params {
     // graph replay: when set, the step number a kernel receives is relative
     // and *nbase is added to it on the device (sources, receiver rows)
     const int *nbase;
+    // variable density, radius >= kVdPre, z-marching kernels: the density's
+    // first-derivative sums along z, the middle axis and the contiguous axis
+    // (k_rho_diff), which do not change between steps
+    const real_t *drho[3];
 };
+
+// Radius from which the z-marching variable-density kernels read the
+// density's derivative sums from arrays made once per run instead of
+// forming them from 2R density neighbours per axis at every step.
+constexpr int kVdPre = 6;
 
 // ---------------------------------------------------------------- helpers
 
@@ -228,6 +237,50 @@ __global__ void k_prepare(const real_t *vel, const real_t *damp,
         e[p] = damp[p] * dt / (2 * s);
     }
 }
+
+#if SW_VD
+// Density derivative sums of the variable-density stencil at every interior
+// point: d[0] along z, d[1] along the middle axis (3D: x, stride sx), d[2]
+// along the contiguous axis (3D: y; 2D: x), each
+// sum_{m=1..R} c1[m] (rho[p + m s] - rho[p - m s]) in the reference's order
+// (vd-3d wave.c:175-186, vd-2d wave.c:160-170), starting from zero, so the
+// kernels that read them form the same bits as from the neighbours.
+template <int R>
+__global__ void k_rho_diff(Params P, real_t *__restrict__ dz, real_t *__restrict__ dm,
+                           real_t *__restrict__ dc)
+{
+    const long long ny = SW_DIM == 3 ? P.ny : 1;
+    const long long iy = SW_DIM == 3 ? P.ny - 2 * R : 1;
+    const long long ix = P.nx - 2 * R, iz = P.nz - 2 * R;
+    const long long cnt = iz * ix * iy;
+    const real_t *__restrict__ rho = P.rho;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long rem = t;
+        const long long k = SW_DIM == 3 ? R + rem % iy : 0;
+        rem = SW_DIM == 3 ? rem / iy : rem;
+        const long long j = R + rem % ix;
+        const long long i = R + rem / ix;
+        const long long p = i * P.sz + j * (SW_DIM == 3 ? P.sx : 1) + (SW_DIM == 3 ? k : 0);
+        (void)ny;
+        real_t az = 0, am = 0, ac = 0;
+#pragma unroll
+        for (int m = 1; m <= R; m++) {
+            az += P.c1[m] * (rho[p + m * P.sz] - rho[p - m * P.sz]);
+#if SW_DIM == 3
+            am += P.c1[m] * (rho[p + m * P.sx] - rho[p - m * P.sx]);
+            ac += P.c1[m] * (rho[p + m] - rho[p - m]);
+#else
+            ac += P.c1[m] * (rho[p + m] - rho[p - m]);
+#endif
+        }
+        dz[p] = az;
+        if (dm)
+            dm[p] = am;
+        dc[p] = ac;
+    }
+}
+#endif
 
 // C = (dt2/s)/den with den = (float)(1.0 + e): the reference's per-point
 // factor `(dtSquared / slowness) / denominator` (cd-3d wave.c:184), which
@@ -497,6 +550,8 @@ __device__ __forceinline__ void step3d_z_block(const Params &P, const LapDiv &D,
     const long long col = (long long)j * sx + k;
     real_t qz[2 * R + 1];
 #if SW_VD
+    // PRE: the density derivative sums come from P.drho (kVdPre)
+    constexpr bool PRE = R >= kVdPre;
     const real_t *__restrict__ rho = P.rho;
     real_t rz[2 * R + 1];
 #endif
@@ -504,7 +559,8 @@ __device__ __forceinline__ void step3d_z_block(const Params &P, const LapDiv &D,
     for (int m = 0; m < 2 * R; m++) {
         qz[m + 1] = uc[(long long)(z0 - R + m) * sz + col];
 #if SW_VD
-        rz[m + 1] = rho[(long long)(z0 - R + m) * sz + col];
+        if constexpr (!PRE)
+            rz[m + 1] = rho[(long long)(z0 - R + m) * sz + col];
 #endif
     }
     struct Ops {
@@ -513,6 +569,7 @@ __device__ __forceinline__ void step3d_z_block(const Params &P, const LapDiv &D,
 #if SW_VD
         real_t rhead;
         real_t ryn[2 * R], rxn[2 * R];
+        real_t r0, dz, dm, dc;
 #endif
     };
     auto load = [&](int i, Ops &o) {
@@ -526,13 +583,20 @@ __device__ __forceinline__ void step3d_z_block(const Params &P, const LapDiv &D,
             o.xn[2 * m - 1] = uc[p - m * sx];
         }
 #if SW_VD
-        o.rhead = rho[p + R * sz];
+        if constexpr (PRE) {
+            o.r0 = rho[p];
+            o.dz = P.drho[0][p];
+            o.dm = P.drho[1][p];
+            o.dc = P.drho[2][p];
+        } else {
+            o.rhead = rho[p + R * sz];
 #pragma unroll
-        for (int m = 1; m <= R; m++) {
-            o.ryn[2 * m - 2] = rho[p + m];
-            o.ryn[2 * m - 1] = rho[p - m];
-            o.rxn[2 * m - 2] = rho[p + m * sx];
-            o.rxn[2 * m - 1] = rho[p - m * sx];
+            for (int m = 1; m <= R; m++) {
+                o.ryn[2 * m - 2] = rho[p + m];
+                o.ryn[2 * m - 1] = rho[p - m];
+                o.rxn[2 * m - 2] = rho[p + m * sx];
+                o.rxn[2 * m - 1] = rho[p - m * sx];
+            }
         }
 #endif
         o.up = up[p];
@@ -548,12 +612,14 @@ __device__ __forceinline__ void step3d_z_block(const Params &P, const LapDiv &D,
         for (int m = 0; m < 2 * R; m++) {
             qz[m] = qz[m + 1];
 #if SW_VD
-            rz[m] = rz[m + 1];
+            if constexpr (!PRE)
+                rz[m] = rz[m + 1];
 #endif
         }
         qz[2 * R] = cur.head;
 #if SW_VD
-        rz[2 * R] = cur.rhead;
+        if constexpr (!PRE)
+            rz[2 * R] = cur.rhead;
 #endif
         const long long p = (long long)i * sz + col;
         const real_t c = qz[R];
@@ -572,21 +638,28 @@ __device__ __forceinline__ void step3d_z_block(const Params &P, const LapDiv &D,
             acc_z += P.c2[m] * (qz[R + m] + qz[R - m]);
 #if SW_VD
             du_c += P.c1[m] * (cp - cm);
-            dr_c += P.c1[m] * (cur.ryn[2 * m - 2] - cur.ryn[2 * m - 1]);
             du_m += P.c1[m] * (mp - mm);
-            dr_m += P.c1[m] * (cur.rxn[2 * m - 2] - cur.rxn[2 * m - 1]);
             du_z += P.c1[m] * (qz[R + m] - qz[R - m]);
-            dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+            if constexpr (!PRE) {
+                dr_c += P.c1[m] * (cur.ryn[2 * m - 2] - cur.ryn[2 * m - 1]);
+                dr_m += P.c1[m] * (cur.rxn[2 * m - 2] - cur.rxn[2 * m - 1]);
+                dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+            }
 #endif
         }
         real_t value = 0;
         value += lap_quotients(P, D, acc_c, acc_m, acc_z);
 #if SW_VD
         {
+            if constexpr (PRE) {
+                dr_c = cur.dc;
+                dr_m = cur.dm;
+                dr_z = cur.dz;
+            }
             const real_t t_c = (du_c * dr_c) / (4 * P.dy2);
             const real_t t_m = (du_m * dr_m) / (4 * P.dx2);
             const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
-            value -= (t_c + t_m + t_z) / rz[R];
+            value -= (t_c + t_m + t_z) / (PRE ? cur.r0 : rz[R]);
         }
 #endif
         real_t v = leapfrog(value, cur.q, cur.e, c, cur.up);
@@ -738,6 +811,8 @@ __device__ __forceinline__ void step2d_z_block(const Params &P, const real_t *__
     const long long sz = P.sz;
     real_t qz[2 * R + 1];
 #if SW_VD
+    // PRE: the density derivative sums come from P.drho (kVdPre)
+    constexpr bool PRE = R >= kVdPre;
     const real_t *__restrict__ rho = P.rho;
     real_t rz[2 * R + 1];
 #endif
@@ -745,7 +820,8 @@ __device__ __forceinline__ void step2d_z_block(const Params &P, const real_t *__
     for (int m = 0; m < 2 * R; m++) {
         qz[m + 1] = uc[(long long)(z0 - R + m) * sz + j];
 #if SW_VD
-        rz[m + 1] = rho[(long long)(z0 - R + m) * sz + j];
+        if constexpr (!PRE)
+            rz[m + 1] = rho[(long long)(z0 - R + m) * sz + j];
 #endif
     }
     struct Ops {
@@ -754,6 +830,7 @@ __device__ __forceinline__ void step2d_z_block(const Params &P, const real_t *__
 #if SW_VD
         real_t rhead;
         real_t rxn[2 * R];
+        real_t r0, dz, dc;
 #endif
     };
     auto load = [&](int i, Ops &o) {
@@ -765,11 +842,17 @@ __device__ __forceinline__ void step2d_z_block(const Params &P, const real_t *__
             o.xn[2 * m - 1] = uc[p - m];
         }
 #if SW_VD
-        o.rhead = rho[p + R * sz];
+        if constexpr (PRE) {
+            o.r0 = rho[p];
+            o.dz = P.drho[0][p];
+            o.dc = P.drho[2][p];
+        } else {
+            o.rhead = rho[p + R * sz];
 #pragma unroll
-        for (int m = 1; m <= R; m++) {
-            o.rxn[2 * m - 2] = rho[p + m];
-            o.rxn[2 * m - 1] = rho[p - m];
+            for (int m = 1; m <= R; m++) {
+                o.rxn[2 * m - 2] = rho[p + m];
+                o.rxn[2 * m - 1] = rho[p - m];
+            }
         }
 #endif
         o.up = up[p];
@@ -785,12 +868,14 @@ __device__ __forceinline__ void step2d_z_block(const Params &P, const real_t *__
         for (int m = 0; m < 2 * R; m++) {
             qz[m] = qz[m + 1];
 #if SW_VD
-            rz[m] = rz[m + 1];
+            if constexpr (!PRE)
+                rz[m] = rz[m + 1];
 #endif
         }
         qz[2 * R] = cur.head;
 #if SW_VD
-        rz[2 * R] = cur.rhead;
+        if constexpr (!PRE)
+            rz[2 * R] = cur.rhead;
 #endif
         const long long p = (long long)i * sz + j;
         const real_t c = qz[R];
@@ -806,18 +891,24 @@ __device__ __forceinline__ void step2d_z_block(const Params &P, const real_t *__
             acc_z += P.c2[m] * (qz[R + m] + qz[R - m]);
 #if SW_VD
             du_c += P.c1[m] * (cp - cm);
-            dr_c += P.c1[m] * (cur.rxn[2 * m - 2] - cur.rxn[2 * m - 1]);
             du_z += P.c1[m] * (qz[R + m] - qz[R - m]);
-            dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+            if constexpr (!PRE) {
+                dr_c += P.c1[m] * (cur.rxn[2 * m - 2] - cur.rxn[2 * m - 1]);
+                dr_z += P.c1[m] * (rz[R + m] - rz[R - m]);
+            }
 #endif
         }
         real_t value = 0;
         value += acc_c / P.dx2 + acc_z / P.dz2;
 #if SW_VD
         {
+            if constexpr (PRE) {
+                dr_c = cur.dc;
+                dr_z = cur.dz;
+            }
             const real_t t_c = (du_c * dr_c) / (4 * P.dx2);
             const real_t t_z = (du_z * dr_z) / (4 * P.dz2);
-            value -= (t_c + t_z) / rz[R];
+            value -= (t_c + t_z) / (PRE ? cur.r0 : rz[R]);
         }
 #endif
         real_t v = leapfrog(value, cur.q, cur.e, c, cur.up);

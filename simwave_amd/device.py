@@ -13,7 +13,7 @@ import ctypes
 
 import numpy as np
 
-from .middleware import Middleware
+from .middleware import KEY_ORDER, Middleware
 
 __all__ = ["DeviceForward", "FLAG_TIME_STEPS", "FLAG_GENERIC", "FLAG_OVERLAP_RECEIVERS",
            "FLAG_GRAPH"]
@@ -95,7 +95,7 @@ class DeviceForward:
                                    % (key, tuple(self.tensors[key].shape), host.shape))
         self.kwargs = kwargs
         argtypes, args = Middleware.ordered(kwargs)
-        keys = [k for k in Middleware._keys_in_order_static() if kwargs.get(k) is not None]
+        keys = [k for k in KEY_ORDER if kwargs.get(k) is not None]
         for pos, key in enumerate(keys):
             if key in self.tensors:
                 argtypes[pos] = ctypes.c_void_p

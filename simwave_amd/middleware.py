@@ -129,24 +129,9 @@ class Middleware:
             return _CTYPE["np({})".format(value.dtype)]
         return _CTYPE[type(value).__name__]
 
-    @property
-    def _keys_in_order(self):
-        return list(KEY_ORDER)
-
-    @staticmethod
-    def _keys_in_order_static():
-        return list(KEY_ORDER)
-
-    def _argtypes(self, **kwargs):
-        return {k: self._ctype_of(v) for k, v in kwargs.items()}
-
     @staticmethod
     def _convert_boundary_condition(boundary_condition):
         return np.uint([_BC_CODES[b] for b in boundary_condition])
-
-    @staticmethod
-    def _convert_type_to_ctypes(type_name):
-        return _CTYPE[type_name]
 
 
 def _last_error(lib):

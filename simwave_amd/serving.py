@@ -47,7 +47,8 @@ def percentile(values, q):
 def shot_bytes(solver):
     """HBM an admitted shot holds: wavefield slots, models, shot record and
     the library's per-point arrays (q, damping term, (dt^2/s)/den, density
-    copy)."""
+    copy and, for variable density from space order 12, the density
+    derivative sums)."""
     if hasattr(solver, "device_bytes"):           # test doubles
         return float(solver.device_bytes)
     sm = solver.space_model
@@ -55,8 +56,11 @@ def shot_bytes(solver):
     esz = np.dtype(sm.dtype).itemsize
     slots = solver.num_snapshots + 2
     models = 3 if sm.density_model is not None else 2
+    # variable density from space order 12: the density derivative sums
+    sums = len(sm.extended_shape) if (sm.density_model is not None
+                                       and sm.space_order >= 12) else 0
     rec = solver.time_model.timesteps * solver.receivers.count
-    return float((slots + models + 4) * npts * esz + rec * esz)
+    return float((slots + models + 4 + sums) * npts * esz + rec * esz)
 
 
 class ServeStats:

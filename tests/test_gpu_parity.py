@@ -62,6 +62,15 @@ CASES = {
     "3d_cd_f64_so8": dict(dim=3, n=26, space_order=8, dtype=F64, tf=0.08),
     "3d_vd_f64_so4": dict(dim=3, n=26, space_order=4, density=True, dtype=F64,
                           tf=0.08),
+    # variable density from space order 12: density derivative sums made
+    # once per run (the z-marching kernels)
+    "2d_vd_so12": dict(dim=2, space_order=12, density=True, n=44),
+    "2d_vd_so20_bc": dict(dim=2, space_order=20, density=True, n=52, damping=0.0,
+                          bc=("null_neumann", "null_dirichlet", "null_neumann", "none")),
+    "3d_vd_so12": dict(dim=3, n=34, space_order=12, density=True, tf=0.05),
+    "3d_vd_so20_bc": dict(dim=3, n=40, space_order=20, density=True, damping=0.0,
+                          bc=("null_dirichlet", "null_neumann", "null_neumann", "none",
+                              "null_dirichlet", "null_neumann"), tf=0.04),
     "3d_vd_f64_so12": dict(dim=3, n=30, space_order=12, density=True, dtype=F64,
                            tf=0.05),
     "3d_bc_neumann": dict(dim=3, n=26, bc=("null_neumann",) * 6, damping=0.0,
@@ -261,6 +270,7 @@ def test_host_entry_keeps_initial_halo_values(gpu, name):
 ZMARCH = ["3d_cd_so2", "3d_cd_so8", "3d_cd_so20", "3d_cd_f64_so8", "3d_bc_neumann",
           "3d_bc_mixed", "3d_src_halo", "3d_wide_bc", "3d_deep_so8",
           "3d_vd_so4", "3d_vd_so8", "3d_vd_f64_so4", "3d_vd_bc", "3d_vd_so16_wide",
+          "3d_vd_f64_so12", "2d_vd_f64_so16", "3d_vd_so20_bc",
           "2d_cd_so2", "2d_cd_so8", "2d_cd_so20", "2d_vd_so4", "2d_vd_so16", "2d_cd_f64_so8",
           "2d_vd_f64_so8", "2d_bc_mixed", "2d_bc_neumann", "2d_src_halo", "2d_src_overlap",
           "2d_stride3"]
