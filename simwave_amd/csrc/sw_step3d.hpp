@@ -111,8 +111,8 @@ __device__ __forceinline__ unsigned long long damp_bits(const Tiling &T, int col
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 
 // A buffer offset past the end of every descriptor the tiled kernel builds
-// (their ranges stay below 2^31 bytes): loads return zeros, stores are
-// dropped.
+// (their ranges stay below 2^31 bytes), also with a plane offset (below
+// 2^31) added: loads return zeros, stores are dropped.
 constexpr unsigned kOOB = 0x80000000u;
 
 __device__ __forceinline__ rsrc_t make_rsrc(const void *base, unsigned long long bytes)
@@ -129,9 +129,17 @@ __device__ __forceinline__ vreal vload(rsrc_t r, unsigned voff, unsigned soff)
     return __builtin_bit_cast(vreal, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
+// The store adds the plane offset into the VGPR offset and passes a zero
+// SGPR offset.  A 128-bit buffer store reads its data VGPRs after issue: an
+// instruction that overwrites them right behind it needs wait states, which
+// the compiler inserts only for stores without an SGPR offset (the ISA's
+// wait-state table exempts those).  On gfx950 an SGPR-offset store followed
+// at once by a write of its data registers stored the new values (seen as
+// byte offsets in the wavefield); with the zero SGPR offset the wait states
+// are there.
 __device__ __forceinline__ void vstore(vreal v, rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), r, voff + soff, 0u, 0);
 }
 
 __device__ __forceinline__ void sstore(real_t v, rsrc_t r, unsigned voff, unsigned soff)
@@ -352,15 +360,37 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
             smask |= 1ull << s;
     }
 
-    // A clean tile touches no boundary face and no source box, and each of
+    // A clean tile touches no Neumann face and no source box, and each of
     // its lanes holds a vector that is wholly interior or wholly outside.
     // Its march issues the same memory instructions on every plane (loads
     // that are not needed and stores of lanes outside go to kOOB, which the
     // descriptor range check turns into zeros / dropped writes), so the
     // compiler's wait counts only ever wait for the data a plane consumes.
+    // A Dirichlet face only zeroes the interior plane next to it (its
+    // phase mirrors nothing), so a clean tile applies the Dirichlet faces
+    // it touches itself: zeros stored where i, j or k is that plane (the
+    // reference's zeroing, wave.c:304-480).  The z planes are tile-uniform
+    // (zpb, zpa); the x rows and y columns (dxy) take extra stores in a
+    // branch: per-element selects of the stored vectors took the kernel
+    // above 256 VGPRs, one wavefront per SIMD.
     const bool lanes_whole = ((ny - 2 * R) % V == 0) || (k0 + TY <= ny - R);
-    const bool clean = !bc_xy && smask == 0 && !(bz_b && z0 <= r2) &&
-                       !(bz_a && z1 - 1 >= nz - r2 - 1) && lanes_whole;
+    bool nm_xy = false;     // a Neumann x or y face within 2R
+    if ((bc >> 8 & 3) == 2 && k0 <= r2)
+        nm_xy = true;
+    if ((bc >> 10 & 3) == 2 && k0 + TY - 1 >= ny - r2 - 1)
+        nm_xy = true;
+    if ((bc >> 4 & 3) == 2 && j0 <= r2)
+        nm_xy = true;
+    if ((bc >> 6 & 3) == 2 && j0 + TXR - 1 >= nx - r2 - 1)
+        nm_xy = true;
+    const bool dxy = ((bc >> 4 & 3) == 1 && j0 <= r2) ||
+                     ((bc >> 6 & 3) == 1 && j0 + TXR - 1 >= nx - r2 - 1) ||
+                     ((bc >> 8 & 3) == 1 && k0 <= r2) ||
+                     ((bc >> 10 & 3) == 1 && k0 + TY - 1 >= ny - r2 - 1);
+    const int zpb = (bc & 3) == 1 ? R : -1;
+    const int zpa = (bc >> 2 & 3) == 1 ? nz - R - 1 : -1;
+    const bool clean = !nm_xy && smask == 0 && !((bc & 3) == 2 && z0 <= r2) &&
+                       !((bc >> 2 & 3) == 2 && z1 - 1 >= nz - r2 - 1) && lanes_whole;
 
     // The march is instantiated twice, for clean and other tiles, with
     // everything it keeps in registers declared inside, so the two share
@@ -375,10 +405,10 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
 #pragma unroll
         for (int q = 0; q < RPT; q++)
             qz[q][m + 1] = vload(r_uc, cb[q] + (unsigned)m * PB, 0u);
-    // One register set per prefetched quantity: the centre plane, its halos,
-    // u_prev, C and the damping term of plane i+1 are all loaded right after
-    // plane i is written to LDS (plane i's update operands move to their own
-    // registers first).
+    // One register set per prefetched quantity: the centre plane and its
+    // halos for plane i+1 are loaded right after plane i is written to LDS;
+    // u_prev, C and the damping term for plane i+1 right after plane i's
+    // update has consumed them.
     vreal nc[RPT], np[RPT], nq[RPT], ne[RPT], nhx[HX], nhy[HY];
     const unsigned long long dbits = damp_bits(T, col, z0);
     auto load_centre = [&](int I) {
@@ -436,9 +466,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     };
     // The scheduler barriers keep the prologue's loads in the loop's order
     // (centre and halos before the update operands), so the wait counts
-    // merged at the loop header stay exact: the halo writes of a plane then
-    // wait only for the halo loads, not for the update operands issued
-    // after them.
+    // merged at the loop header stay exact.
     if constexpr (CLEAN) {
         load_centre_clean(z0);
         __builtin_amdgcn_sched_barrier(0);
@@ -473,24 +501,11 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
 #pragma unroll
         for (int h = 0; h < HY; h++)
             *(vreal *)&lb[yh_lds[h]] = nhy[h];
-        // this plane's update operands (loaded during the previous plane),
-        // then every load of the next plane, so that each load has a whole
-        // plane's computation to arrive
-        vreal cp[RPT], cq[RPT], ce[RPT];
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            cp[q] = np[q];
-            cq[q] = nq[q];
-            ce[q] = ne[q];
-        }
         const bool more = i + 1 < z1;
-        if constexpr (CLEAN) {
+        if constexpr (CLEAN)
             load_centre_clean(i + 1);
-            load_update_clean(i + 1);
-        } else if (more) {
+        else if (more)
             load_centre(i + 1);
-            load_update(i + 1);
-        }
         __syncthreads();
 
         const bool bc_here =
@@ -591,13 +606,13 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                 real_t value = 0;
                 value += lap[e];
 #if defined(SW_DOUBLE)
-                out[q][e] = damped ? leapfrog_damped(value, cq[q][e], ce[q][e], qz[q][R][e],
-                                                     cp[q][e])
-                                   : leapfrog_plain(value, cq[q][e], qz[q][R][e], cp[q][e]);
+                out[q][e] = damped ? leapfrog_damped(value, nq[q][e], ne[q][e], qz[q][R][e],
+                                                     np[q][e])
+                                   : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
 #else
-                out[q][e] = damped ? leapfrog_damped_rcp(value, cq[q][e], ce[q][e],
-                                                         qz[q][R][e], cp[q][e])
-                                   : leapfrog_plain(value, cq[q][e], qz[q][R][e], cp[q][e]);
+                out[q][e] = damped ? leapfrog_damped_rcp(value, nq[q][e], ne[q][e],
+                                                         qz[q][R][e], np[q][e])
+                                   : leapfrog_plain(value, nq[q][e], qz[q][R][e], np[q][e]);
 #endif
             }
             if (!CLEAN && special && act[q]) {
@@ -620,10 +635,33 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                 }
             }
         }
+        // stores first, then the next plane's update operands, so that
+        // waiting for those never waits for the stores
         if constexpr (CLEAN) {
+            if (i == zpb || i == zpa) {
+#pragma unroll
+                for (int q = 0; q < RPT; q++)
+                    out[q] = (vreal)(real_t)0;
+            }
 #pragma unroll
             for (int q = 0; q < RPT; q++)
                 vstore(out[q], r_un, st_off[q], so);
+            if (dxy) {
+                // the Dirichlet x rows and y columns the tile holds: zeros
+                // over the values just stored (stores of one lane to one
+                // address complete in program order)
+#pragma unroll
+                for (int q = 0; q < RPT; q++) {
+                    const int j = j0 + slot * RPT + q;
+                    if (((bc >> 4 & 3) == 1 && j == R) || ((bc >> 6 & 3) == 1 && j == nx - R - 1))
+                        vstore((vreal)(real_t)0, r_un, st_off[q], so);
+                    if (act[q] && (bc >> 8 & 3) == 1 && k <= R && R < k + V)
+                        sstore((real_t)0, r_un, cb[q] + (unsigned)(R - k) * esz, so);
+                    if (act[q] && (bc >> 10 & 3) == 1 && k <= ny - R - 1 && ny - R - 1 < k + V)
+                        sstore((real_t)0, r_un, cb[q] + (unsigned)(ny - R - 1 - k) * esz, so);
+                }
+            }
+            load_update_clean(i + 1);
         } else {
             if (!special) {
 #pragma unroll
@@ -640,6 +678,8 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
                     }
                 }
             }
+            if (more)
+                load_update(i + 1);
         }
     }
     };

@@ -110,6 +110,14 @@ CASES = {
                         dtype=F64, tf=0.03),
     "3d_deep_so4": dict(dim=3, shape=(52, 30, 260), space_order=4, damping=30.0,
                         tf=0.04),
+    # Dirichlet on every face: the tiled kernel's clean tiles zero the face
+    # planes by selects, corners and edges included
+    "3d_deep_dirichlet": dict(dim=3, shape=(58, 30, 301), space_order=8, damping=30.0,
+                              bc=("null_dirichlet",) * 6, tf=0.04),
+    # no boundary condition and 12 damping points: clean tiles march over
+    # planes that hold damping (the damped update and its operand loads)
+    "3d_deep_damped": dict(dim=3, shape=(60, 30, 301), space_order=8, damping=120.0,
+                           bc=("none",) * 6, tf=0.04),
     # spacings whose squares B 2^t (B odd) have t <= 0 (12.5^2 = 625/4,
     # 15^2 = 225): the float32 tiled kernel forms the Laplacian quotients
     # as float64 products with RN(1/h^2)
@@ -175,7 +183,8 @@ def _tiled(name, tile, order):
         os.environ.pop("SIMWAVE_HIP_ORDER", None)
 
 
-DEEP_R4 = ["3d_deep_so8", "3d_deep_so8_whole", "3d_deep_none", "3d_deep_f64", "3d_deep_h12p5"]
+DEEP_R4 = ["3d_deep_so8", "3d_deep_so8_whole", "3d_deep_none", "3d_deep_f64", "3d_deep_h12p5",
+           "3d_deep_dirichlet", "3d_deep_damped"]
 
 
 @pytest.mark.parametrize("order", [0, 1])
