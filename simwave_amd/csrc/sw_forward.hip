@@ -19,6 +19,7 @@
 #include "sw_adjoint.hpp"
 #include "sw_batch.hpp"
 
+#include <hip/hip_ext.h>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -68,6 +69,25 @@ constexpr size_t kScratchBytes = 64ull << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
+
+// Timing of one step-kernel launch: events that the runtime sets at the
+// kernel's own start and end (hipExtLaunchKernelGGL), so a timed run puts
+// no marker packets between the kernels and reads what a profiler reads
+// (marker pairs around a 30 us launch added about 6 us).  Both null: an
+// ordinary launch.
+struct LaunchTimer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+template <typename F, typename... Args>
+void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t st, LaunchTimer tm,
+                  Args... args)
+{
+    if (tm.e0)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, tm.e0, tm.e1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
 
 using sw::Box;
 using sw::Params;
@@ -358,7 +378,7 @@ struct Prepared {
     sw::LapDiv lapdiv{};         // its fixed-divisor quotients
     bool tiled = false;          // use k_step3d_vec
     void (*tiled_launch)(const struct Prepared &, const real_t *, const real_t *,
-                         real_t *, int, hipStream_t) = nullptr;
+                         real_t *, int, hipStream_t, LaunchTimer) = nullptr;
     const Params *Pd = nullptr;   // device copy of P for the tiled kernel
     const real_t *qc = nullptr;   // (dt2/s)/den per point for the tiled kernel
     bool fastdiv = false;
@@ -715,10 +735,11 @@ void set_lap_div(Prepared &pr)
 // thread RPT, wavefronts per block NW).  Radius 4 (space order 8) carries
 // every configuration; other radii carry 0, 4 and 8.  SIMWAVE_HIP_TILE picks
 // one; otherwise default_tile(R).  Measured per launch on one MI355X
-// (float32): radius 4 is fastest with 8 rows x 64 points and 8-plane z
-// chunks; radii 5-10 with 8 rows x 32 points and 4-plane chunks (10-13%
-// ahead of 4-wavefront tiles at 312^3-376^3).  Float64 keeps its measured
-// radius-4 setting (12-plane chunks) everywhere.
+// (float32): radius 4 is fastest with 8 rows x 64 points and 6-plane z
+// chunks (472^3: 532 us, 550 us with 8 planes, 540 us with 4); radii 5-10
+// with 8 rows x 32 points and 4-plane chunks (10-13% ahead of 4-wavefront
+// tiles at 312^3-376^3).  Float64: the same tile, 8-plane chunks at radius
+// 4 (472^3: 884 us, 891 us with 12), 12-plane chunks elsewhere.
 constexpr int kTileShapes = 17;
 constexpr int default_tile(int R)
 {
@@ -726,7 +747,7 @@ constexpr int default_tile(int R)
 }
 constexpr int default_tile_zlen(int R)
 {
-    return sizeof(real_t) == 4 ? (R == 4 ? 8 : R > 4 ? 4 : 12) : 12;
+    return sizeof(real_t) == 4 ? (R == 4 ? 6 : R > 4 ? 4 : 12) : (R == 4 ? 8 : 12);
 }
 
 // Whether the z-marching one-point-per-thread kernel beats the tiled one
@@ -758,7 +779,7 @@ bool prefer_zmarch(const Prepared &pr)
 
 template <int R, int WR, int RPT, int NW, bool D64>
 void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un,
-                  int n, hipStream_t st)
+                  int n, hipStream_t st, LaunchTimer tm)
 {
     const Params &P = pr.P;
     sw::StepArgs A{};
@@ -791,12 +812,12 @@ void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t
     A.nsrc = P.nsrc;
     A.sbox = P.sbox;
     A.T = pr.tiling;
-    hipLaunchKernelGGL((sw::k_step3d_vec<R, WR, RPT, NW, D64>), dim3(pr.tiled_blocks),
-                       dim3(64, NW), 0, st, A, pr.Pd, n);
+    launch_timed(sw::k_step3d_vec<R, WR, RPT, NW, D64>, dim3(pr.tiled_blocks), dim3(64, NW),
+                 st, tm, A, pr.Pd, n);
 }
 
 using TiledFn = void (*)(const Prepared &, const real_t *, const real_t *, real_t *, int,
-                         hipStream_t);
+                         hipStream_t, LaunchTimer);
 struct TiledKernel {
     TiledFn launch;
     TiledFn launch64;   // float64-reciprocal quotients (float32 builds)
@@ -941,7 +962,7 @@ constexpr int kTX = 8;
 
 template <int R>
 void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *un,
-                  int n, hipStream_t st)
+                  int n, hipStream_t st, LaunchTimer tm)
 {
     const int Iy = P.ny - 2 * R, Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
     const int gx = (Iy + 63) / 64, gy = (Ix + kTX - 1) / kTX;
@@ -953,8 +974,8 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
         zlen = (Iz + nzc - 1) / nzc;
     }
     const int gz = (Iz + zlen - 1) / zlen;
-    hipLaunchKernelGGL((sw::k_step3d<R, kTX>), dim3(gx, gy, gz), dim3(64, kTX), 0, st,
-                       P, uc, up, un, n, zlen);
+    launch_timed(sw::k_step3d<R, kTX>, dim3(gx, gy, gz), dim3(64, kTX), st, tm, P, uc, up, un,
+                 n, zlen);
 }
 #else
 constexpr int kTB = 256;
@@ -985,7 +1006,7 @@ int zlen_2d(const Params &P, int R)
 
 template <int R>
 void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *un,
-                  int n, hipStream_t st)
+                  int n, hipStream_t st, LaunchTimer tm)
 {
     const int Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
     const int gx = (Ix + kTB - 1) / kTB;
@@ -999,12 +1020,12 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
     // k_step2d, SIMWAVE_HIP_ZMARCH the z-marching one.
     if (zmarch_2d(R)) {
         static_assert(sw::kZ2Block == kTB, "2D launch shapes differ");
-        hipLaunchKernelGGL(sw::k_step2d_z<R>, dim3(gx, gz), dim3(sw::kZ2Block), 0, st,
-                           P, uc, up, un, n, zlen);
+        launch_timed(sw::k_step2d_z<R>, dim3(gx, gz), dim3(sw::kZ2Block), st, tm, P, uc, up,
+                     un, n, zlen);
         return;
     }
-    hipLaunchKernelGGL((sw::k_step2d<R, kTB>), dim3(gx, gz), dim3(kTB), 0, st,
-                       P, uc, up, un, n, zlen);
+    launch_timed(sw::k_step2d<R, kTB>, dim3(gx, gz), dim3(kTB), st, tm, P, uc, up, un, n,
+                 zlen);
 }
 #endif
 
@@ -1024,7 +1045,7 @@ void launch_generic(const Params &P, const real_t *uc, const real_t *up, real_t 
 }
 
 using StepFn = void (*)(const Params &, const real_t *, const real_t *, real_t *,
-                        int, hipStream_t);
+                        int, hipStream_t, LaunchTimer);
 using GenFn = void (*)(const Params &, const real_t *, const real_t *, real_t *,
                        hipStream_t);
 
@@ -1090,7 +1111,7 @@ struct RecStream {
 #if SW_DIM == 3
 // One step of k_step3d_z over the interior.
 void zmarch_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un, int n,
-                   hipStream_t st)
+                   hipStream_t st, LaunchTimer tm)
 {
     const Params &P = pr.P;
     const int R = pr.r, zl = pr.zmarch_zlen;
@@ -1101,7 +1122,7 @@ void zmarch_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_
     switch (R) {
 #define SW_Z_CASE(RR)                                                                   \
     case RR:                                                                            \
-        hipLaunchKernelGGL(sw::k_step3d_z<RR>, grid, block, 0, st, P, pr.lapdiv, uc, up, un, n, zl); \
+        launch_timed(sw::k_step3d_z<RR>, grid, block, st, tm, P, pr.lapdiv, uc, up, un, n, zl); \
         return;
     SW_Z_CASE(1) SW_Z_CASE(2) SW_Z_CASE(3) SW_Z_CASE(4) SW_Z_CASE(5)
     SW_Z_CASE(6) SW_Z_CASE(7) SW_Z_CASE(8) SW_Z_CASE(9) SW_Z_CASE(10)
@@ -1124,9 +1145,11 @@ void zmarch_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_
 // it before any later step whose output slot is cur(n), before step n+2 in
 // any case, before uploads into recycled buffers and before slot exchanges.
 //
-// With kFlagTimeSteps every step kernel is bracketed by its own event pair
-// on `st` and the durations are summed after the loop, so the measurement
-// adds no synchronisation between steps.
+// With kFlagTimeSteps every fused step kernel is launched with its own
+// event pair, set by the runtime at the kernel's start and end (the generic
+// path's stencil kernel is bracketed by event records instead), and the
+// durations are summed after the loop, so the measurement adds no
+// synchronisation between steps.
 template <typename SlotPtr, typename Before, typename After, typename Swap,
           typename Uploads>
 void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &plan,
@@ -1214,18 +1237,15 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
             if (pr.halo_sources)
                 hipLaunchKernelGGL(sw::k_inject, dim3(1), dim3(256), 0, st, P, un,
                                    (int)n, 1);
-            if (timed)
-                hip_check(hipEventRecord(t0, st), "hipEventRecord");
+            const LaunchTimer tm{t0, t1};
 #if SW_DIM == 3
             if (pr.zmarch)
-                zmarch_launch(pr, uc, up, un, (int)n, st);
+                zmarch_launch(pr, uc, up, un, (int)n, st, tm);
             else if (pr.tiled)
-                pr.tiled_launch(pr, uc, up, un, (int)n, st);
+                pr.tiled_launch(pr, uc, up, un, (int)n, st, tm);
             else
 #endif
-                fused(P, uc, up, un, (int)n, st);
-            if (timed)
-                hip_check(hipEventRecord(t1, st), "hipEventRecord");
+                fused(P, uc, up, un, (int)n, st, tm);
         } else {
             if (timed)
                 hip_check(hipEventRecord(t0, st), "hipEventRecord");
@@ -2161,27 +2181,27 @@ double run_device_noalloc(const Args &a, hipStream_t st, unsigned flags)
 // the whole-grid k_adj_update for radii they do not carry.
 #if SW_DIM == 3
 template <int R>
-void adj_update_z(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *ub,
-                  const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
+void adj_update_z(const sw::Params &P, hipStream_t st, LaunchTimer tm, const real_t *vel,
+                  real_t *ub, const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
                   const real_t *u1, const real_t *u2, real_t *gacc, int image)
 {
     const dim3 grid((unsigned)((P.ny + 63) / 64),
                     (unsigned)((P.nx + sw::kAdjLines - 1) / sw::kAdjLines),
                     (unsigned)((P.nz + sw::kAdjZChunk - 1) / sw::kAdjZChunk));
-    hipLaunchKernelGGL(sw::k_adj_update_z<R>, grid, sw::adj_block(), 0, st, P, vel, ub, W,
-                       ub1, ub2, u0, u1, u2, gacc, image);
+    launch_timed(sw::k_adj_update_z<R>, grid, sw::adj_block(), st, tm, P, vel, ub, W, ub1, ub2,
+                 u0, u1, u2, gacc, image);
 }
 #endif
 
-void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *ub,
-                const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
+void adj_update(const sw::Params &P, hipStream_t st, LaunchTimer tm, const real_t *vel,
+                real_t *ub, const real_t *W, real_t *ub1, real_t *ub2, const real_t *u0,
                 const real_t *u1, const real_t *u2, real_t *gacc, int image)
 {
 #if SW_DIM == 3
     switch (P.r) {
 #define SW_ADJ_CASE(R)                                                              \
     case R:                                                                         \
-        adj_update_z<R>(P, st, vel, ub, W, ub1, ub2, u0, u1, u2, gacc, image);      \
+        adj_update_z<R>(P, st, tm, vel, ub, W, ub1, ub2, u0, u1, u2, gacc, image);  \
         return;
     SW_ADJ_CASE(1) SW_ADJ_CASE(2) SW_ADJ_CASE(3) SW_ADJ_CASE(4) SW_ADJ_CASE(5)
     SW_ADJ_CASE(6) SW_ADJ_CASE(7) SW_ADJ_CASE(8) SW_ADJ_CASE(9) SW_ADJ_CASE(10)
@@ -2193,10 +2213,10 @@ void adj_update(const sw::Params &P, hipStream_t st, const real_t *vel, real_t *
     switch (P.r) {
 #define SW_ADJ_CASE(R)                                                              \
     case R:                                                                         \
-        hipLaunchKernelGGL(sw::k_adj_update_z2d<R>,                                \
-                           dim3((unsigned)((P.nx + 255) / 256),                     \
-                                (unsigned)((P.nz + sw::kAdjZChunk - 1) / sw::kAdjZChunk)), \
-                           dim3(256), 0, st, P, vel, ub, W, ub1, ub2, u0, u1, u2, gacc, image); \
+        launch_timed(sw::k_adj_update_z2d<R>,                                      \
+                     dim3((unsigned)((P.nx + 255) / 256),                           \
+                          (unsigned)((P.nz + sw::kAdjZChunk - 1) / sw::kAdjZChunk)), \
+                     dim3(256), st, tm, P, vel, ub, W, ub1, ub2, u0, u1, u2, gacc, image); \
         return;
     SW_ADJ_CASE(1) SW_ADJ_CASE(2) SW_ADJ_CASE(3) SW_ADJ_CASE(4) SW_ADJ_CASE(5)
     SW_ADJ_CASE(6) SW_ADJ_CASE(7) SW_ADJ_CASE(8) SW_ADJ_CASE(9) SW_ADJ_CASE(10)
@@ -2526,14 +2546,13 @@ double run_adjoint(const Args &a, const real_t *rec_adj, double *grad, real_t *g
                 hipLaunchKernelGGL(sw::k_adj_sources, dim3((unsigned)a.nsrc), dim3(256), 0, st,
                                    P, ub, gw + (n - 1) * a.nsrc);
             const bool timed = utime.used + 2 <= utime.ev.size();
-            if (timed)
-                hip_check(hipEventRecord(utime.ev[utime.used], st), "hipEventRecord");
-            adj_update(P, st, a.vel, ub, W, ub1, ub2, seg + (n + 1 - m) * npts,
-                       seg + (n - m) * npts, seg + (n - 1 - m) * npts, gacc, image ? 1 : 0);
+            LaunchTimer tm;
             if (timed) {
-                hip_check(hipEventRecord(utime.ev[utime.used + 1], st), "hipEventRecord");
+                tm = LaunchTimer{utime.ev[utime.used], utime.ev[utime.used + 1]};
                 utime.used += 2;
             }
+            adj_update(P, st, tm, a.vel, ub, W, ub1, ub2, seg + (n + 1 - m) * npts,
+                       seg + (n - m) * npts, seg + (n - 1 - m) * npts, gacc, image ? 1 : 0);
             if (++reverse_steps % kGradFlush == 0)
                 flush_grad();
             hip_check(hipGetLastError(), "adjoint step");

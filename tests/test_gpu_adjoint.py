@@ -202,3 +202,23 @@ def test_gradient_is_deterministic(name, dtype):
     bound = 1e-12 if dtype == F64 else 1e-5
     assert harness.rel_l2(a["grad_velocity"], c["grad_velocity"]) < bound
     assert harness.rel_l2(a["grad_wavelet"], c["grad_wavelet"]) < bound
+
+
+@pytest.mark.parametrize("name", ["2d_multi", "3d_so8"])
+def test_timed_reverse_updates_keep_the_gradient(name):
+    """Timing the reverse update kernels (entry flag 8: runtime-set start
+    and end events on each launch) leaves the misfit and gradient bits
+    unchanged and counts one launch per reverse step."""
+    import torch
+    print("timed reverse updates", name, flush=True)
+    _, solver, kw = cases.adjoint_shot(name, dtype=F32)
+    rng = np.random.default_rng(5)
+    obs = rng.standard_normal((kw["end_timestep"], kw["num_receivers"])).astype(F32)
+    dev = DeviceForward(solver)
+    m0, g0 = adjoint.shot_misfit_gradient(dev, obs, checkpoint_every=5)
+    m1, g1 = adjoint.shot_misfit_gradient(dev, obs, checkpoint_every=5, time_update=True)
+    t = dev.timings()
+    torch.cuda.synchronize()
+    assert m0 == m1
+    assert torch.equal(g0, g1)
+    assert t[5] == kw["end_timestep"] and t[4] > 0.0, t   # one per reverse step

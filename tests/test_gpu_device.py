@@ -55,3 +55,27 @@ def test_shot_built_on_one_stream_runs_on_another(name):
     u, rec = dev.results(other)
     dev.release_plan()
     assert np.array_equal(u, u_ref) and np.array_equal(rec, rec_ref)
+
+
+@pytest.mark.parametrize("name,env", [("3d_cd", {"SIMWAVE_HIP_TILE": "10"}),
+                                      ("3d_cd", {"SIMWAVE_HIP_ZMARCH": "1"}),
+                                      ("2d_cd", {}), ("2d_vd_f64", {}), ("3d_vd", {})])
+def test_timed_steps_keep_results(name, env, monkeypatch):
+    """Step timing (flag 1) launches the step kernels with runtime-set
+    start and end events; the results are the untimed run's bits, the
+    launch count is the step count and every launch took some time."""
+    print("timed steps", name, env, flush=True)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    solver = sw.Solver(**cases.make_case(**CASES[name])())
+    dev = DeviceForward(solver)
+    dev.run(flags=0)
+    u0, r0 = dev.results()
+    for flags in (1, 1 | 4):
+        dev.reset()
+        dev.run(flags=flags)
+        t = dev.timings()
+        u, r = dev.results()
+        assert np.array_equal(u, u0) and np.array_equal(r, r0), flags
+        assert t[5] == dev.steps and t[4] > 0.0, (flags, t)
+        assert t[4] / t[5] < 0.05, t   # seconds per launch of a tiny grid
