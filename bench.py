@@ -60,6 +60,10 @@ WORKLOADS = {
                       "shots, 3D so8 160x256x256 (228x324x324 extended), 15 m, tf 1.2 s, "
                       "100x100 receivers, checkpoints every 20 steps",
     "serving": "mixed shot stream through the continuous-batching server",
+    "3d_vd_so8_256": "3D variable density so8 256^3, 12.5 m, 10-pt damping, tf 0.3 s, "
+                     "64x64 receivers (profiling grid of the variable-density kernel)",
+    "3d_vd_so16_256": "3D variable density so16 256^3, 12.5 m, 10-pt damping, tf 0.3 s, "
+                      "64x64 receivers (profiling grid of the variable-density kernel)",
 }
 
 
@@ -169,6 +173,8 @@ def cpu_workload(workload, dtype, seed):
         parts = workloads.faulted_2d_vd(dtype)
     elif workload == "2d_cd_so4_readme":
         parts = workloads.readme_2d(dtype)
+    elif workload.startswith("3d_vd_so"):
+        parts = workloads.vd_3d(int(workload.split("_")[2][2:]), dtype)
     else:
         parts = workloads.headline_3d(dtype)
     kw = Middleware.normalize(**sw.Solver(**parts).forward_arguments())
@@ -530,7 +536,7 @@ def run_serving(args):
             delay = due - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            futs.append(srv.submit(lambda r=r: build(r), arrival=due))
+            futs.append(srv.submit(lambda r=r: build(r), arrival=due, keep_result=False))
         for i, f in enumerate(futs):
             f.result(timeout=900)
             if (i + 1) % 50 == 0 or i + 1 == len(futs):
@@ -672,6 +678,9 @@ def main():
         out = run_host_workload(args, workloads.faulted_2d_vd(dtype))
     elif args.workload == "2d_cd_so4_readme":
         out = run_device_workload(args, workloads.readme_2d(dtype))
+    elif args.workload.startswith("3d_vd_so"):
+        out = run_device_workload(args, workloads.vd_3d(int(args.workload.split("_")[2][2:]),
+                                                        dtype))
     else:
         out = run_device_workload(args, workloads.headline_3d(dtype))
     print(json.dumps(out), flush=True)

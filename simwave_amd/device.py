@@ -10,6 +10,7 @@ PyTorch's caching allocator (plumbing only; every computation is in the HIP
 library).
 """
 import ctypes
+import threading
 
 import numpy as np
 
@@ -35,6 +36,23 @@ def _variant(kwargs):
     dens = "vd" if kwargs.get("density_model") is not None else "cd"
     prec = {"float32": "f32", "float64": "f64"}[str(vel.dtype)]
     return "%dd_%s_%s" % (vel.ndim, dens, prec)
+
+
+_STAGE = threading.local()
+
+
+def _staging(torch, nbytes):
+    """This thread's pinned staging buffer (uint8) of at least `nbytes`,
+    grown in powers of two: the D2H copies of results() run at pinned
+    speed, and a long-running server holds one buffer per worker thread
+    instead of one per result kept by its callers."""
+    buf = getattr(_STAGE, "buf", None)
+    if buf is None or buf.numel() < nbytes:
+        _STAGE.buf = None
+        size = 1 << max(20, (int(nbytes) - 1).bit_length())
+        buf = torch.empty(size, dtype=torch.uint8, pin_memory=True)
+        _STAGE.buf = buf
+    return buf
 
 
 class DeviceForward:
@@ -272,9 +290,10 @@ class DeviceForward:
     def results(self, stream=None):
         """(u_full, shot_record) as the Solver returns them.  The returned
         snapshots (one slot for saving_stride 0) are cut out of the slots on
-        the device, so only they cross PCIe, straight into pinned host
-        memory, on `stream` (default: torch's current stream), which is
-        synchronised before returning."""
+        the device, so only they cross PCIe, into this thread's pinned
+        staging buffer, on `stream` (default: torch's current stream), which
+        is synchronised; the arrays returned are ordinary (pageable) copies,
+        so results a caller keeps hold no page-locked memory."""
         torch = self._torch
         sm, tm = self.solver.space_model, self.solver.time_model
         if stream is None:
@@ -294,9 +313,12 @@ class DeviceForward:
             u = sm.remove_halo_region(slots)
             u = u.contiguous()
             rec = self.tensors["shot_record"]
-            u_h = torch.empty(u.shape, dtype=u.dtype, pin_memory=True)
-            r_h = torch.empty(rec.shape, dtype=rec.dtype, pin_memory=True)
+            nu = u.numel() * u.element_size()
+            nr = rec.numel() * rec.element_size()
+            buf = _staging(torch, nu + nr)
+            u_h = buf[:nu].view(u.dtype).view(u.shape)
+            r_h = buf[nu:nu + nr].view(rec.dtype).view(rec.shape)
             u_h.copy_(u, non_blocking=True)
             r_h.copy_(rec, non_blocking=True)
         stream.synchronize()
-        return u_h.numpy(), r_h.numpy()
+        return u_h.numpy().copy(), r_h.numpy().copy()

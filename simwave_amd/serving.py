@@ -107,13 +107,15 @@ class ServeStats:
 
 
 class _Request:
-    __slots__ = ("solver", "arrival", "future", "admitted")
+    __slots__ = ("solver", "arrival", "future", "admitted", "keep_result", "seq")
 
-    def __init__(self, solver, arrival):
+    def __init__(self, solver, arrival, seq=0):
+        self.seq = seq
         self.solver = solver
         self.arrival = arrival
         self.future = Future()
         self.admitted = None
+        self.keep_result = True
 
 
 class ShotServer:
@@ -157,7 +159,8 @@ class ShotServer:
         self._used = 0.0
         self._closed = False
         self._workers = []
-        self.admissions = []          # requests (as submitted) in admission order
+        self.admissions = []          # submission numbers (0, 1, ...) in admission order
+        self._submitted = 0
         self.batch = bool(batch)
         self._runners = {}            # variant -> _BatchRunner
         self.stats = ServeStats()
@@ -166,12 +169,17 @@ class ShotServer:
         self._sched.start()
 
     # ------------------------------------------------------------ public
-    def submit(self, solver, arrival=None):
-        """Queues a shot; returns a Future of (u_full, shot_record)."""
+    def submit(self, solver, arrival=None, keep_result=True):
+        """Queues a shot; returns a Future of (u_full, shot_record).  With
+        keep_result False the results are still fetched (and timed) but the
+        Future holds None, for callers that only measure the stream."""
         req = _Request(solver, time.perf_counter() if arrival is None else arrival)
+        req.keep_result = bool(keep_result)
         with self._cv:
             if self._closed:
                 raise RuntimeError("server is closed")
+            req.seq = self._submitted
+            self._submitted += 1
             self._waiting.append(req)
             self._cv.notify_all()
         return req.future
@@ -236,13 +244,14 @@ class ShotServer:
                         self._active >= self.max_active:
                     self._cv.wait()
                 self._waiting.popleft()
-                self.admissions.append(req.solver)
+                self.admissions.append(req.seq)
                 self._used += need
                 self._active += 1
                 active_now = self._active
                 req.admitted = time.perf_counter()
             t = threading.Thread(target=self._work, name="shot-worker", daemon=True,
                                  args=(req, need, active_now))
+            self._workers = [w for w in self._workers if w.is_alive()]
             self._workers.append(t)
             t.start()
 
@@ -281,7 +290,7 @@ class ShotServer:
                            updates=shot.updates_per_shot, active_at_admission=active_now,
                            prep=t1 - t0, build=tb - t0, plan=t1 - tb, device=t2 - t1,
                            fetch=done - t2, batched=batched)
-            req.future.set_result(result)
+            req.future.set_result(result if req.keep_result else None)
         except Exception as exc:
             if shot is not None:
                 try:

@@ -16,12 +16,14 @@ the stated structure (there are no field datasets here):
   15 m, space order 8, 30-cell damping, a layered true model with a
   spherical +10% anomaly, its sigma-8 Gaussian smoothing as the starting
   model, 16 sources on a 4 x 4 surface grid, 100 x 100 receivers;
-* ``serving_requests``: a seeded stream of mixed 2D/3D requests.
+* ``serving_requests``: a seeded stream of mixed 2D/3D requests;
+* ``vd_3d``: 3D variable density, 256^3 at 12.5 m, space order 8 or 16,
+  the profiling grid of the variable-density step kernel.
 """
 import numpy as np
 
 __all__ = ["readme_2d", "headline_3d", "faulted_2d_vd", "fwi_models", "fwi_shot",
-           "serving_requests", "interior_updates"]
+           "serving_requests", "vd_3d", "interior_updates"]
 
 
 def interior_updates(solver_or_kwargs):
@@ -121,6 +123,35 @@ def faulted_2d_vd(dtype=np.float32, seed=0, tf=3.0):
                                        np.linspace(0.0, (nx - 1) * h, 3000)])
     return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
                 wavelet=sw.RickerWavelet(25.0, tm))
+
+
+def vd_3d(space_order=8, dtype=np.float32, seed=0, n=256, tf=0.3):
+    """3D variable density, n^3 at 12.5 m, 10-cell cubic damping; 6 flat
+    layers 1500..4500 m/s with seeded +-5% cell noise, density from
+    Gardner's relation rho = 310 v^0.25; a 15 Hz Ricker at 100 m depth in
+    the centre; 64 x 64 receivers at 25 m depth; Neumann surface,
+    Dirichlet elsewhere; saving_stride 0."""
+    import simwave_amd as sw
+    h = 12.5
+    rng = np.random.default_rng(seed)
+    layers = np.linspace(1500.0, 4500.0, 6)
+    vel = np.repeat(layers, -(-n // 6))[:n].astype(np.float64)[:, None, None] * (
+        1.0 + 0.05 * rng.uniform(-1.0, 1.0, size=(n, n, n)))
+    den = (310.0 * vel ** 0.25).astype(dtype)
+    vel = vel.astype(dtype)
+    ext = (n - 1) * h
+    sm = sw.SpaceModel(bounding_box=(0, ext) * 3, grid_spacing=(h, h, h),
+                       velocity_model=vel, density_model=den, space_order=space_order,
+                       dtype=dtype)
+    sm.config_boundary(damping_length=10 * h,
+                       boundary_condition=("null_neumann",) + ("null_dirichlet",) * 5,
+                       damping_polynomial_degree=3, damping_alpha=0.001)
+    tm = sw.TimeModel(space_model=sm, tf=tf, saving_stride=0)
+    src = sw.Source(sm, coordinates=[(100.0, ext / 2, ext / 2)])
+    g = np.linspace(0.0, ext, 64)
+    rec = sw.Receiver(sm, coordinates=[(25.0, x, y) for x in g for y in g])
+    return dict(space_model=sm, time_model=tm, sources=src, receivers=rec,
+                wavelet=sw.RickerWavelet(15.0, tm))
 
 
 # FWI experiment geometry

@@ -84,7 +84,7 @@ def test_budget_and_cap_are_respected():
     assert mon.peak_used <= 45
     assert mon.peak_active <= 3
     # admitted in arrival order (the workers may start in either order)
-    assert [r.ident for r in srv.admissions] == [r.ident for r in reqs]
+    assert srv.admissions == list(range(len(reqs)))
     assert sorted(mon.order) == [r.ident for r in reqs]
     s = srv.stats.summary()
     assert s["requests"] == 40 and s["max_active"] <= 3
@@ -161,3 +161,17 @@ def test_percentile_matches_numpy_linear():
     vals = [5.0, 1.0, 3.0, 2.0, 4.0]
     assert percentile(vals, 50) == 3.0
     assert abs(percentile(vals, 99) - 4.96) < 1e-12
+
+
+def test_results_dropped_on_request():
+    """keep_result=False: the shot still runs and its results are fetched
+    (timed in the stats), but the Future holds None."""
+    mon = Monitor()
+    reqs = [FakeRequest(i, nbytes=1, steps=10) for i in range(6)]
+    with make_server(mon, hbm_budget=100, max_active=4, chunk_steps=4) as srv:
+        futs = [srv.submit(r, keep_result=(r.ident % 2 == 0)) for r in reqs]
+    for r, f in zip(reqs, futs):
+        got = f.result(timeout=30)
+        assert got == ((("u", r.ident), ("rec", r.ident)) if r.ident % 2 == 0 else None)
+    s = srv.stats.summary()
+    assert s["requests"] == 6 and s["updates"] == sum(1000 * r.steps for r in reqs)
