@@ -634,7 +634,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=200,
                     help="serving: requests of the stream run one at a time for the "
                          "plain and device-resident paths")
-    ap.add_argument("--rate", type=float, default=20.0, help="serving arrivals per second")
+    ap.add_argument("--rate", type=float, default=30.0,
+                    help="serving arrivals per second (the default stream of 2000 "
+                         "requests is sustainable at it: about 85 Gpts/s offered)")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving budget, GB")
     ap.add_argument("--max-active", type=int, default=32)
     ap.add_argument("--chunk-steps", type=int, default=48)

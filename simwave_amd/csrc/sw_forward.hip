@@ -1027,6 +1027,26 @@ void launch_fused(const Params &P, const real_t *uc, const real_t *up, real_t *u
     launch_timed(sw::k_step2d<R, kTB>, dim3(gx, gz), dim3(kTB), st, tm, P, uc, up, un, n,
                  zlen);
 }
+
+// k_step2d_z with the receiver gather of level n in the same launch (the
+// 2D z-marching path with receivers on the step stream); 64-product rounds
+// keep the gather's LDS at about 4 KB per block.
+constexpr int kRecChunkFused = 64;
+
+template <int R>
+void launch_fused_rec(const Params &P, const real_t *uc, const real_t *up, real_t *un, int n,
+                      hipStream_t st, LaunchTimer tm, const Box *rbox, const real_t *rw,
+                      int nrec, real_t *row)
+{
+    const int Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    const int gx = (Ix + kTB - 1) / kTB;
+    const int zlen = zlen_2d(P, R);
+    const int gz = (Iz + zlen - 1) / zlen;
+    const int nrb = (nrec + kRecPerBlock - 1) / kRecPerBlock;
+    launch_timed(sw::k_step2d_z_rec<R, kRecPerBlock, kRecChunkFused>,
+                 dim3((unsigned)(nrb + gx * gz)), dim3(sw::kZ2Block), st, tm, P, uc, up, un, n,
+                 zlen, gx, nrb, rbox, rw, nrec, row);
+}
 #endif
 
 template <int R>
@@ -1049,10 +1069,19 @@ using StepFn = void (*)(const Params &, const real_t *, const real_t *, real_t *
 using GenFn = void (*)(const Params &, const real_t *, const real_t *, real_t *,
                        hipStream_t);
 
+#if SW_DIM == 2
+using StepRecFn = void (*)(const Params &, const real_t *, const real_t *, real_t *, int,
+                           hipStream_t, LaunchTimer, const Box *, const real_t *, int,
+                           real_t *);
+#endif
+
 template <int... Rs>
 struct Tables {
     static constexpr StepFn fused[] = {nullptr, &launch_fused<Rs>...};
     static constexpr GenFn generic[] = {nullptr, &launch_generic<Rs>...};
+#if SW_DIM == 2
+    static constexpr StepRecFn fused_rec[] = {nullptr, &launch_fused_rec<Rs>...};
+#endif
 };
 using RTable = Tables<1, 2, 3, 4, 5, 6, 7, 8, 9, 10>;
 
@@ -1199,6 +1228,13 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
     };
     const unsigned rec_blocks = (unsigned)((a.nrec + 255) / 256);
     const bool coop_receivers = !env_flag("SIMWAVE_HIP_REC_THREAD");
+#if SW_DIM == 2
+    // 2D z-marching steps carry their receiver gather (one launch per step)
+    const bool rec_in_step = pr.fused && a.nrec && !overlap_rec && coop_receivers &&
+                             zmarch_2d(r) && !env_flag("SIMWAVE_HIP_REC_SEPARATE");
+#else
+    const bool rec_in_step = false;
+#endif
     const unsigned coop_blocks = (unsigned)((a.nrec + kRecPerBlock - 1) / kRecPerBlock);
     auto receivers = [&](size_t n, const real_t *uc, hipStream_t s) {
         real_t *row = rec_dev + (n - 1 - rec_row0) * a.nrec;
@@ -1244,6 +1280,11 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
             else if (pr.tiled)
                 pr.tiled_launch(pr, uc, up, un, (int)n, st, tm);
             else
+#else
+            if (rec_in_step)
+                RTable::fused_rec[r](P, uc, up, un, (int)n, st, tm, pr.rbox, pr.rw,
+                                     (int)a.nrec, rec_dev + (n - 1 - rec_row0) * a.nrec);
+            else
 #endif
                 fused(P, uc, up, un, (int)n, st, tm);
         } else {
@@ -1257,7 +1298,7 @@ void time_loop(const Args &a, const Prepared &pr, const std::vector<StepSlots> &
                                    (int)n, 0);
             launch_bc_phases(P, un, st);
         }
-        if (a.nrec && !overlap_rec)
+        if (a.nrec && !overlap_rec && !rec_in_step)
             receivers(n, uc, st);
         hip_check(hipGetLastError(), "step launch");
         if (s.swap_after) {

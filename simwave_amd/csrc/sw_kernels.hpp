@@ -1268,6 +1268,34 @@ k_receivers_coop(Params P, const real_t *__restrict__ uc, const Box *__restrict_
     receivers_coop_block<RPB, CH>(P, uc, rbox, rw, nrec, out, blockIdx.x);
 }
 
+#if SW_DIM == 2
+// One 2D step and its receiver gather in one launch: blocks 0 .. nrb-1
+// gather the receivers of level n (they only read u^n, which the step does
+// not write), the rest are k_step2d_z's blocks (gx along x).  Saves the
+// second launch per step of small 2D grids, whose steps take a few
+// microseconds.  The gather's chunk CH only sets how many products a round
+// stages in LDS (small here, so the step blocks keep their occupancy); the
+// sum per receiver runs in the same order for any CH.
+template <int R, int RPB, int CH>
+__global__ void __launch_bounds__(kZ2Block)
+k_step2d_z_rec(Params P, const real_t *__restrict__ uc, const real_t *__restrict__ up,
+               real_t *__restrict__ un, int n, int zlen, int gx, int nrb,
+               const Box *__restrict__ rbox, const real_t *__restrict__ rw, int nrec,
+               real_t *__restrict__ out)
+{
+    static_assert(kZ2Block == 256, "the gather's block shape");
+    const int b = blockIdx.x;
+    if (b < nrb) {
+        if (P.nbase)
+            out += (long long)(*P.nbase) * nrec;
+        receivers_coop_block<RPB, CH>(P, uc, rbox, rw, nrec, out, b);
+        return;
+    }
+    const int s = b - nrb;
+    step2d_z_block<R>(P, uc, up, un, n, zlen, s % gx, s / gx);
+}
+#endif
+
 // Zero-fills a word array (a kernel rather than a memset, so it behaves the
 // same inside captured graphs).
 // The step base of a graph replay (a kernel, so that no memset node is
