@@ -1047,6 +1047,66 @@ void launch_fused_rec(const Params &P, const real_t *uc, const real_t *up, real_
                  dim3((unsigned)(nrb + gx * gz)), dim3(sw::kZ2Block), st, tm, P, uc, up, un, n,
                  zlen, gx, nrb, rbox, rw, nrec, row);
 }
+
+// Steps n0 .. n1 of a stride-0 2D shot in one cooperative launch
+// (k_step2d_persist) of at most two blocks per compute unit (and at most
+// the blocks of one step).  bar: two words, the barrier count and the
+// abort word, zeroed here.  Polls per barrier before a block gives up: far
+// beyond any step's time, so only a missing arrival ends a run early; the
+// roll call that opens the launch gives up sooner, before anything is
+// written.
+constexpr int kPersistSpin = 1 << 22;
+constexpr int kPersistRollSpin = 1 << 17;
+constexpr int kPersistBlocksPerCU = 2;
+
+template <int R>
+void launch_persist(const Params &P, real_t *u, long long np, int n0, int n1, hipStream_t st,
+                    const Box *rbox, const real_t *rw, int nrec, real_t *rec0, unsigned *bar)
+{
+    const auto kernel = sw::k_step2d_persist<R, kRecPerBlock, kRecChunkFused>;
+    static int per_cu = -1, ncu = 0;
+    static std::mutex mu;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (per_cu < 0) {
+            int dev = 0;
+            hip_check(hipGetDevice(&dev), "hipGetDevice");
+            hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev),
+                      "hipDeviceGetAttribute");
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, sw::kZ2Block,
+                                                                   0),
+                      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        }
+    }
+    const int Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
+    int gx = (Ix + kTB - 1) / kTB;
+    int zlen = zlen_2d(P, R);
+    int nsb = gx * ((Iz + zlen - 1) / zlen);
+    int nrb = (nrec + kRecPerBlock - 1) / kRecPerBlock;
+    const int bpc = std::max(1, env_int("SIMWAVE_HIP_PERSIST_BPC", kPersistBlocksPerCU));
+    const long long resident = (long long)std::min(per_cu, bpc) * ncu;
+    if (resident < 1)
+        fail("no block of the whole-run 2D kernel fits on a compute unit");
+    const unsigned grid = (unsigned)std::min<long long>(resident, (long long)nsb + nrb);
+    // the barrier count reaches (steps + 1) x grid within the launch
+    if ((long long)(n1 - n0 + 2) * grid >= (1ll << 31))
+        fail("%d steps are too many for one whole-run launch", n1 - n0 + 1);
+    Params Pk = P;
+    int spin = kPersistSpin, roll = kPersistRollSpin;
+    // test knob: a roll call that cannot complete, to exercise the fallback
+    int roll_extra = env_flag("SIMWAVE_HIP_PERSIST_TEST_FALLBACK") ? 1 : 0;
+    if (roll_extra)
+        roll = 1 << 10;
+    int a0 = n0, a1 = n1;
+    unsigned *count = bar;
+    int *abort_flag = reinterpret_cast<int *>(bar + 1);
+    hipLaunchKernelGGL(sw::k_zero_u32, dim3(1), dim3(64), 0, st, bar, 2ll);
+    void *args[] = {&Pk, &u, &np, &a0, &a1, &zlen, &gx, &nsb, &nrb, &rbox, &rw, &nrec, &rec0,
+                    &count, &abort_flag, &spin, &roll, &roll_extra};
+    hip_check(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kernel), dim3(grid),
+                                         dim3(sw::kZ2Block), args, 0, st),
+              "cooperative launch");
+}
 #endif
 
 template <int R>
@@ -1073,6 +1133,8 @@ using GenFn = void (*)(const Params &, const real_t *, const real_t *, real_t *,
 using StepRecFn = void (*)(const Params &, const real_t *, const real_t *, real_t *, int,
                            hipStream_t, LaunchTimer, const Box *, const real_t *, int,
                            real_t *);
+using PersistFn = void (*)(const Params &, real_t *, long long, int, int, hipStream_t,
+                           const Box *, const real_t *, int, real_t *, unsigned *);
 #endif
 
 template <int... Rs>
@@ -1081,6 +1143,7 @@ struct Tables {
     static constexpr GenFn generic[] = {nullptr, &launch_generic<Rs>...};
 #if SW_DIM == 2
     static constexpr StepRecFn fused_rec[] = {nullptr, &launch_fused_rec<Rs>...};
+    static constexpr PersistFn persist[] = {nullptr, &launch_persist<Rs>...};
 #endif
 };
 using RTable = Tables<1, 2, 3, 4, 5, 6, 7, 8, 9, 10>;
@@ -1803,6 +1866,9 @@ struct Plan {
     // runs only on its own
     const sw::BatchShot *bshot = nullptr;
     int bblocks = 0, rblocks = 0;
+    // whole-run 2D launches (k_step2d_persist): the barrier counter and the
+    // abort word; null when the plan runs step by step
+    unsigned *bar = nullptr;
     Plan(const Args &args, hipStream_t s, unsigned f) : a(args), flags(f), st(s), arena(s) {}
     ~Plan()
     {
@@ -1884,7 +1950,7 @@ void capture_graph(Plan &p)
     hip_check(hipGraphInstantiate(&p.gexec, p.graph, nullptr, nullptr, 0), "graph instantiate");
 }
 
-size_t run_plan(Plan &p, size_t count, hipStream_t st);
+size_t run_plan(Plan &p, size_t count, hipStream_t st, bool whole_run = false);
 
 // A plan can join batched runs when its steps use the fused z-marching
 // kernel of a stride-0 run with no source in the halo and no per-step
@@ -2037,6 +2103,32 @@ void run_plans_batched(Plan *const *plans, int count, size_t steps, hipStream_t 
     }
 }
 
+#if SW_DIM == 2
+// Whether a plan may run all its steps in one cooperative launch
+// (k_step2d_persist, run_device only): stride 0 on the fused z-marching 2D
+// kernel with the cooperative receiver gather, no halo-reaching sources, no
+// graph and no second stream, on grids up to kPersistMaxPoints, where a
+// step takes a few microseconds and launching it costs as much.
+// SIMWAVE_HIP_PERSIST=0 switches it off, =1 takes it on any such grid.
+constexpr long long kPersistMaxPoints = 1ll << 21;
+
+void setup_persist(Plan &p)
+{
+    const char *env = getenv("SIMWAVE_HIP_PERSIST");
+    const int want = env ? atoi(env) : -1;
+    if (want == 0)
+        return;
+    const Prepared &pr = p.pr;
+    const bool ok = p.a.stride == 0 && pr.fused && !pr.halo_sources && zmarch_2d(pr.r) &&
+                    !p.gexec && !(p.flags & kFlagOverlapReceivers) &&
+                    !env_flag("SIMWAVE_HIP_REC_THREAD") && p.a.n0 >= 1 &&
+                    p.slots.size() < (1u << 21) && pr.P.nbase == nullptr;
+    if (!ok || (want != 1 && pr.npts > kPersistMaxPoints))
+        return;
+    p.bar = p.arena.alloc<unsigned>(2);
+}
+#endif
+
 Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
 {
     validate(a);
@@ -2059,6 +2151,9 @@ Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
         p->slots.size() >= kGraphSteps && periodic_slots(p->slots))
         capture_graph(*p);
     setup_batch(*p);
+#if SW_DIM == 2
+    setup_persist(*p);
+#endif
     hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventRecord(p->last, st), "hipEventRecord");
     return p.release();
@@ -2066,10 +2161,43 @@ Plan *make_plan(const Args &a, hipStream_t st, unsigned flags)
 
 // Enqueues the next `count` steps of the plan on `st` (after all earlier
 // work of the plan, whatever stream it ran on); returns the steps left.
-size_t run_plan(Plan &p, size_t count, hipStream_t st)
+// whole_run: the caller waits for the stream afterwards and checks the
+// plan's abort word (run_device), so a 2D plan set up for it may run the
+// steps in one cooperative launch.
+size_t run_plan(Plan &p, size_t count, hipStream_t st, bool whole_run)
 {
     const size_t i0 = p.done, i1 = std::min(p.slots.size(), i0 + count);
     hip_check(hipStreamWaitEvent(st, p.last, 0), "hipStreamWaitEvent");
+#if SW_DIM == 2
+    if (whole_run && p.bar && i1 > i0) {
+        const bool timed = p.flags & kFlagTimeSteps;
+        hipEvent_t e[2] = {nullptr, nullptr};
+        if (timed)
+            for (hipEvent_t &x : e) {
+                hip_check(hipEventCreate(&x), "hipEventCreate");
+                hip_check(hipEventRecord(x, st), "hipEventRecord");
+            }
+        const long long n0 = (long long)(p.a.n0 + i0), n1 = (long long)(p.a.n0 + i1 - 1);
+        RTable::persist[p.pr.r](p.pr.P, p.a.u, p.pr.npts, (int)n0, (int)n1, st, p.pr.rbox,
+                                p.pr.rw, (int)p.a.nrec,
+                                p.a.receivers + (n0 - 1) * (long long)p.a.nrec, p.bar);
+        if (timed) {
+            hip_check(hipEventRecord(e[1], st), "hipEventRecord");
+            hip_check(hipEventSynchronize(e[1]), "timed steps");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, e[0], e[1]), "hipEventElapsedTime");
+            g_timings[4] = ms * 1e-3;
+            g_timings[5] = (double)(i1 - i0);
+            for (hipEvent_t x : e)
+                (void)hipEventDestroy(x);
+        }
+        hip_check(hipEventRecord(p.last, st), "hipEventRecord");
+        p.done = i1;
+        g_timings[6] = 1.0;
+        g_timings[7] = 3.0;
+        return p.slots.size() - p.done;
+    }
+#endif
     const size_t npts = (size_t)p.pr.npts;
     real_t *u = p.a.u;
     auto slot_ptr = [&](size_t sl) { return u + sl * npts; };
@@ -2122,8 +2250,23 @@ double run_device(const Args &a, hipStream_t st, unsigned flags)
         t = 0.0;
     std::unique_ptr<Plan> p(make_plan(a, st, flags));
     const double tl0 = now_s();
-    run_plan(*p, p->slots.size(), st);
+    run_plan(*p, p->slots.size(), st, true);
     hip_check(hipStreamSynchronize(st), "time loop");
+    if (p->bar) {
+        int aborted = 0;
+        hip_check(hipMemcpy(&aborted, p->bar + 1, sizeof(int), hipMemcpyDeviceToHost),
+                  "abort word");
+        if (aborted == 1) {
+            // not every block was resident (the device was shared): nothing
+            // was written, so the steps run launch by launch instead
+            p->done = 0;
+            run_plan(*p, p->slots.size(), st, false);
+            hip_check(hipStreamSynchronize(st), "time loop");
+        } else if (aborted) {
+            fail("the whole-run 2D launch gave up waiting at a grid barrier; run with "
+                 "SIMWAVE_HIP_PERSIST=0");
+        }
+    }
     g_timings[2] = now_s() - tl0;
     const double total = now_s() - t0;
     g_timings[0] = total;

@@ -1294,6 +1294,84 @@ k_step2d_z_rec(Params P, const real_t *__restrict__ uc, const real_t *__restrict
     const int s = b - nrb;
     step2d_z_block<R>(P, uc, up, un, n, zlen, s % gx, s / gx);
 }
+
+// Grid-wide barrier of a cooperative launch (every block resident).  Each
+// thread makes its stores visible device-wide (agent-scope fence: the
+// stores complete and the XCD's L2 writes them back), the block's first
+// thread adds one to *count and waits until it reaches `target`, and every
+// thread then fences again (its later loads see the other XCDs' stores).
+// The wait is bounded: after `spin_limit` polls, or once another block has
+// given up, a block stores `code` in *abort (unless a code is there); every
+// block returns false then and leaves the kernel, so no launch can hang on
+// a missing arrival.
+__device__ __forceinline__ bool grid_sync(unsigned *count, unsigned target, int *abort,
+                                          int spin_limit, int code)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(count, 1u);
+        int spins = 0;
+        while (__hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                ++spins > spin_limit) {
+                int none = 0;
+                __hip_atomic_compare_exchange_strong(abort, &none, code, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+    __threadfence();
+    return __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+}
+
+// Steps n0 .. n1 of a stride-0 2D shot in one cooperative launch: per step
+// the blocks take the receiver gather of level n (nrb blocks) and the
+// k_step2d_z blocks (nsb, gx along x) in turn, then meet at a grid barrier
+// before the next step reads what this one wrote.  A roll call opens the
+// launch: if not every block is resident within `roll_spin` polls, all
+// blocks leave before writing anything (*abort = 1) and the caller runs the
+// steps launch by launch instead; a barrier given up later leaves
+// *abort = 2.  Slots follow the
+// stride-0 cycle of the reference's time loop (wave.c:112-124): step n
+// reads levels n-1 and n (slots (n-1) mod 3, n mod 3) and writes level n+1
+// (slot (n+1) mod 3); its receivers go to row n - n0 of rec0.  Same block
+// bodies as the per-step launches, so the same bits.  For small grids,
+// whose steps take a few microseconds, where launching every step costs
+// more than the step.
+template <int R, int RPB, int CH>
+__global__ void __launch_bounds__(kZ2Block)
+k_step2d_persist(Params P, real_t *__restrict__ u, long long np, int n0, int n1, int zlen,
+                 int gx, int nsb, int nrb, const Box *__restrict__ rbox,
+                 const real_t *__restrict__ rw, int nrec, real_t *__restrict__ rec0,
+                 unsigned *count, int *abort, int spin_limit, int roll_spin, int roll_extra)
+{
+    static_assert(kZ2Block == 256, "the gather's block shape");
+    // roll_extra > 0 (tests of the fallback) asks for arrivals that never come
+    if (!grid_sync(count, gridDim.x + roll_extra, abort, roll_spin, 1))
+        return;
+    for (int n = n0; n <= n1; n++) {
+        const real_t *uc = u + (long long)(n % 3) * np;
+        const real_t *up = u + (long long)((n - 1) % 3) * np;
+        real_t *un = u + (long long)((n + 1) % 3) * np;
+        real_t *row = rec0 + (long long)(n - n0) * nrec;
+        for (int b = blockIdx.x; b < nrb + nsb; b += gridDim.x) {
+            if (b < nrb) {
+                receivers_coop_block<RPB, CH>(P, uc, rbox, rw, nrec, row, b);
+            } else {
+                const int s = b - nrb;
+                step2d_z_block<R>(P, uc, up, un, n, zlen, s % gx, s / gx);
+            }
+        }
+        if (!grid_sync(count, (unsigned)(n - n0 + 2) * gridDim.x + roll_extra, abort,
+                       spin_limit, 2))
+            return;
+    }
+}
 #endif
 
 // Zero-fills a word array (a kernel rather than a memset, so it behaves the
