@@ -756,8 +756,10 @@ constexpr int default_tile_zlen(int R)
 // neighbour reads through the caches lose once there are plenty at radius 4.
 // Measured per launch (float32, cubes): radius 1-3 at every size up to
 // 466^3 (1.2-1.9x faster than any tile configuration); radius 4 up to
-// ~50 M interior points (1.3x at 22 M, even at 38 M, tiled 7% ahead at 60 M
-// and 9% at 100 M); radius 5+ never (its register queue is 2R+1 deep).
+// ~27 M interior points (228 x 324 x 324, 22 M: 138 against 149 us; 320^3,
+// 33 M: tiled 196 against 207 us; 384^3, 57 M: 327 against 347 us; with
+// the tiled kernel's 6-plane chunks and Dirichlet faces in its clean
+// march); radius 5+ never (its register queue is 2R+1 deep).
 // Float64: the tiled kernel (z-marching 26% slower at 22 M, radius 4).
 // Variable density: float32 at every radius and size measured, 1.2-1.8x
 // faster than the LDS-tiled k_step3d (176^3-448^3, orders 2-16); float64 at
@@ -774,7 +776,7 @@ bool prefer_zmarch(const Prepared &pr)
     const long long interior = (long long)(P.nz - 2 * R) * (P.nx - 2 * R) * (P.ny - 2 * R);
     if (R <= 3)
         return true;
-    return R == 4 && interior < 50ll * 1000 * 1000;
+    return R == 4 && interior < 27ll * 1000 * 1000;
 }
 
 template <int R, int WR, int RPT, int NW, bool D64>
