@@ -19,6 +19,8 @@
  *     batched launches;
  *   - simwave_hip_forward_device_noalloc_<variant>: the device entry with
  *     every library buffer in a static device array (no allocation);
+ *   - simwave_hip_gather_<variant>: the device gather that builds the
+ *     edge-extended models and damping mask of a device-resident shot;
  *   - simwave_hip_fixed_div_<variant>, simwave_hip_last_error,
  *     simwave_hip_last_timings, simwave_hip_variant, simwave_hip_trim_pool;
  *   - through libsimwave_hip_common.so, which every variant library links:
@@ -160,6 +162,17 @@ int simwave_hip_plans_run_2d_cd_f32(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_2d_cd_f32(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_2d_cd_f32(float *out, const float *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_2d_cd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -323,6 +336,17 @@ int simwave_hip_plans_run_2d_cd_f64(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_2d_cd_f64(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_2d_cd_f64(double *out, const double *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_2d_cd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -487,6 +511,17 @@ int simwave_hip_plans_run_2d_vd_f32(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_2d_vd_f32(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_2d_vd_f32(float *out, const float *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_2d_vd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -651,6 +686,17 @@ int simwave_hip_plans_run_2d_vd_f64(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_2d_vd_f64(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_2d_vd_f64(double *out, const double *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_2d_vd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -814,6 +860,17 @@ int simwave_hip_plans_run_3d_cd_f32(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_3d_cd_f32(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_3d_cd_f32(float *out, const float *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_3d_cd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -977,6 +1034,17 @@ int simwave_hip_plans_run_3d_cd_f64(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_3d_cd_f64(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_3d_cd_f64(double *out, const double *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density).
  * Arguments of simwave_hip_forward_device_3d_cd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -1141,6 +1209,17 @@ int simwave_hip_plans_run_3d_vd_f32(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_3d_vd_f32(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_3d_vd_f32(float *out, const float *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_3d_vd_f32 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the
@@ -1305,6 +1384,17 @@ int simwave_hip_plans_run_3d_vd_f64(void **plans, int count, size_t steps,
 /* 1 when the plan can join batched runs (stride 0, fused z-marching step
  * kernel, no source box in the halo, flags without 1 and 4), else 0. */
 int simwave_hip_plan_batchable_3d_vd_f64(void *plan);
+/* Device gather (no counterpart in the reference; for the device-resident
+ * shot, which builds its edge-extended models and damping mask on the GPU
+ * from their distinct values): out[i][j][k] = small[idx0[i]][idx1[j]][idx2[k]]
+ * for out of out_shape and small of small_shape (host arrays of ndim
+ * entries, ndim the library's dimension; 2D: no k, idx2 ignored); out,
+ * small and the int64 index arrays are device pointers.  Enqueued on
+ * `stream` without waiting.  Returns 0, or -1. */
+int simwave_hip_gather_3d_vd_f64(double *out, const double *small, int ndim,
+                        const long long *out_shape, const long long *small_shape,
+                        const long long *idx0, const long long *idx1,
+                        const long long *idx2, void *stream);
 /* Adjoint-state gradient (no counterpart in the reference; constant density only: this variable-density library returns -1).
  * Arguments of simwave_hip_forward_device_3d_vd_f64 (device arrays; stride 0,
  * steps 1..T, source boxes inside the interior), then: rec_adjoint, the

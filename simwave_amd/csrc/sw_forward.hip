@@ -2949,6 +2949,36 @@ SW_EXPORT int SW_SUFFIX(simwave_hip_plan_batchable_)(void *plan)
     return plan && static_cast<Plan *>(plan)->bshot ? 1 : 0;
 }
 
+SW_EXPORT int SW_SUFFIX(simwave_hip_gather_)(real_t *out, const real_t *small, int ndim,
+                                              const long long *out_shape,
+                                              const long long *small_shape,
+                                              const long long *idx0, const long long *idx1,
+                                              const long long *idx2, void *stream)
+{
+    const double r = guarded([&] {
+        if (ndim != SW_DIM)
+            fail("gather: %d axes for a %dD library", ndim, SW_DIM);
+        if (!out || !small || !out_shape || !small_shape || !idx0 || !idx1 ||
+            (SW_DIM == 3 && !idx2))
+            fail("gather: null argument");
+        long long n[3] = {1, 1, 1}, m[3] = {1, 1, 1};
+        for (int a = 0; a < SW_DIM; a++) {
+            n[a] = out_shape[a];
+            m[a] = small_shape[a];
+            if (n[a] < 1 || m[a] < 1)
+                fail("gather: empty axis %d", a);
+        }
+        const long long total = n[0] * n[1] * n[2];
+        const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(sw::k_gather_axes, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                           out, small, n[0], n[1], n[2], m[1], m[2], idx0, idx1,
+                           SW_DIM == 3 ? idx2 : nullptr);
+        hip_check(hipGetLastError(), "gather launch");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
 SW_EXPORT int SW_SUFFIX(simwave_hip_plan_destroy_)(void *plan)
 {
     const double r = guarded([&] {

@@ -1384,6 +1384,25 @@ __global__ void k_set_int(int *p, int v)
         *p = v;
 }
 
+// out[i][j][k] = small[i0[i]][i1[j]][i2[k]] (2D: no k, i2 null, m2 = 1),
+// out contiguous n0 x n1 x n2 and small m0 x m1 x m2: the edge-extended
+// models and the damping mask built on the device from their distinct
+// values (copies only, so the bits are the host's).
+__global__ void k_gather_axes(real_t *__restrict__ out, const real_t *__restrict__ small,
+                              long long n0, long long n1, long long n2, long long m1,
+                              long long m2, const long long *__restrict__ i0,
+                              const long long *__restrict__ i1,
+                              const long long *__restrict__ i2)
+{
+    const long long total = n0 * n1 * n2;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long k = t % n2, r = t / n2, j = r % n1, i = r / n1;
+        const long long c = i2 ? i2[k] : 0;
+        out[t] = small[(i0[i] * m1 + i1[j]) * m2 + c];
+    }
+}
+
 __global__ void k_zero_u32(unsigned *p, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;

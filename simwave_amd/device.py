@@ -5,9 +5,9 @@ record in HBM and calls the library's device entry
 (``simwave_hip_forward_device_<variant>``, declared in include/simwave_hip.h)
 on a HIP stream, so repeated shots pay no host<->device traffic.  Its plan
 methods use the persistent-plan entries (``simwave_hip_plan_*``) to run a
-shot in chunks of time steps on any stream.  Device memory comes from
-PyTorch's caching allocator (plumbing only; every computation is in the HIP
-library).
+shot in chunks of time steps on any stream.  Device memory, streams and
+pinned staging come from PyTorch (plumbing only); every computation on the
+device, the edge extension of the models included, is in the HIP library.
 """
 import ctypes
 import threading
@@ -151,14 +151,32 @@ class DeviceForward:
         return self._torch.from_numpy(np.ascontiguousarray(array)).to(self.device)
 
     def _gather(self, small, index):
-        """small[np.ix_(*index)] built on the device: one upload of `small`,
-        then an index_select per axis (copies only, so the values are the
-        host's, bit for bit)."""
+        """small[np.ix_(*index)] built on the device: one upload of `small`
+        and of the index arrays, then the library's gather kernel
+        (``simwave_hip_gather_<variant>``) on the current stream (copies
+        only, so the values are the host's, bit for bit)."""
         torch = self._torch
-        t = self._upload(small)
-        for axis, idx in enumerate(index):
-            t = torch.index_select(t, axis, self._upload(idx))
-        return t.contiguous()
+        small = np.ascontiguousarray(small)
+        src = self._upload(small)
+        idx = [self._upload(np.asarray(i, dtype=np.int64)) for i in index]
+        shape = tuple(len(i) for i in index)
+        out = torch.empty(shape, dtype=src.dtype, device=self.device)
+        fn = getattr(self._lib, "simwave_hip_gather_" + self.variant)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                       ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        pad = (1,) * (3 - len(shape))
+        n = (ctypes.c_longlong * 3)(*(shape + pad))
+        m = (ctypes.c_longlong * 3)(*(tuple(small.shape) + pad))
+        ptr = [t.data_ptr() for t in idx] + [None] * (3 - len(idx))
+        stream = torch.cuda.current_stream(self.device)
+        rc = fn(out.data_ptr(), src.data_ptr(), len(shape), n, m, ptr[0], ptr[1], ptr[2],
+                ctypes.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            from .middleware import _last_error
+            raise RuntimeError("device gather failed: {}".format(_last_error(self._lib)))
+        return out
 
     @property
     def device_bytes(self):
