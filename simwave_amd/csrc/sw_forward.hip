@@ -732,18 +732,25 @@ void set_lap_div(Prepared &pr)
 
 #if SW_DIM == 3
 // Tile configurations of k_step3d_vec as (rows per wavefront WR, rows per
-// thread RPT, wavefronts per block NW).  Radius 4 (space order 8) carries
-// every configuration; other radii carry 0, 4 and 8.  SIMWAVE_HIP_TILE picks
-// one; otherwise default_tile(R).  Measured per launch on one MI355X
-// (float32): radius 4 is fastest with 8 rows x 64 points and 6-plane z
-// chunks (472^3: 532 us, 550 us with 8 planes, 540 us with 4); radii 5-10
-// with 8 rows x 32 points and 4-plane chunks (10-13% ahead of 4-wavefront
-// tiles at 312^3-376^3).  Float64: the same tile, 8-plane chunks at radius
-// 4 (472^3: 884 us, 891 us with 12), 12-plane chunks elsewhere.
-constexpr int kTileShapes = 17;
+// thread RPT, wavefronts per block NW, and 16- or 8-byte lanes).  Radius 4
+// (space order 8) carries every configuration; other radii carry 0, 4, 8,
+// 17 and 18.  SIMWAVE_HIP_TILE picks one; otherwise default_tile(R).
+// Measured per launch on one MI355X (float32, 472^3): radius 4 is fastest
+// with 4 rows x 128 points of 8-byte lanes, 4 rows per lane, and 6-plane z
+// chunks (511 us; 521 us with 8 planes; 8 rows x 64 points of 8-byte lanes
+// 515 us; of 16-byte lanes, 2 rows per lane, 532 us).  The 8-byte lanes
+// halve the registers of each z queue entry, so a lane can hold 4 adjacent
+// rows and take more x neighbours from registers; the shapes with 3
+// wavefronts per SIMD (133-137 VGPRs) came to 541-578 us.  Radii 5-10: 8
+// rows x 32 points of 16-byte lanes and 4-plane chunks (10-13% ahead of
+// 4-wavefront tiles at 312^3-376^3; the 8-byte shapes 1-10% behind at 320^3
+// so12 and so16).  Float64: 8 rows x 32 points of 16-byte lanes, 8-plane
+// chunks at radius 4 (472^3: 881 us; the 8-byte shapes 894-1097 us), 12-plane
+// chunks elsewhere.
+constexpr int kTileShapes = 24;
 constexpr int default_tile(int R)
 {
-    return R == 4 ? 10 : R < 4 ? 0 : (sizeof(real_t) == 4 ? 8 : 4);
+    return R == 4 ? (sizeof(real_t) == 4 ? 20 : 10) : R < 4 ? 0 : (sizeof(real_t) == 4 ? 8 : 4);
 }
 constexpr int default_tile_zlen(int R)
 {
@@ -779,7 +786,7 @@ bool prefer_zmarch(const Prepared &pr)
     return R == 4 && interior < 27ll * 1000 * 1000;
 }
 
-template <int R, int WR, int RPT, int NW, bool D64>
+template <int R, int WR, int RPT, int NW, int V, bool D64>
 void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t *un,
                   int n, hipStream_t st, LaunchTimer tm)
 {
@@ -814,7 +821,7 @@ void tiled_launch(const Prepared &pr, const real_t *uc, const real_t *up, real_t
     A.nsrc = P.nsrc;
     A.sbox = P.sbox;
     A.T = pr.tiling;
-    launch_timed(sw::k_step3d_vec<R, WR, RPT, NW, D64>, dim3(pr.tiled_blocks), dim3(64, NW),
+    launch_timed(sw::k_step3d_vec<R, WR, RPT, NW, V, D64>, dim3(pr.tiled_blocks), dim3(64, NW),
                  st, tm, A, pr.Pd, n);
 }
 
@@ -826,13 +833,13 @@ struct TiledKernel {
     int ty, txr;
 };
 
-template <int R, int WR, int RPT, int NW>
+template <int R, int WR, int RPT, int NW, int V = sw::kVec16>
 TiledKernel tiled_config()
 {
-    using S = sw::TileShape<R, WR, RPT, NW>;
+    using S = sw::TileShape<R, WR, RPT, NW, V>;
     constexpr bool f32 = sizeof(real_t) == 4;
-    return TiledKernel{&tiled_launch<R, WR, RPT, NW, false>,
-                       &tiled_launch<R, WR, RPT, NW, f32>, S::TY, S::TXR};
+    return TiledKernel{&tiled_launch<R, WR, RPT, NW, V, false>,
+                       &tiled_launch<R, WR, RPT, NW, V, f32>, S::TY, S::TXR};
 }
 
 template <int R>
@@ -856,12 +863,22 @@ TiledKernel tiled_entry(int v)
         case 14: return tiled_config<R, 8, 2, 1>();
         case 15: return tiled_config<R, 4, 2, 2>();
         case 16: return tiled_config<R, 8, 2, 2>();
+        // 8-byte lanes
+        case 17: return tiled_config<R, 2, 4, 1, sw::kVec8>();
+        case 18: return tiled_config<R, 2, 4, 2, sw::kVec8>();
+        case 19: return tiled_config<R, 4, 4, 1, sw::kVec8>();
+        case 20: return tiled_config<R, 1, 4, 1, sw::kVec8>();
+        case 21: return tiled_config<R, 2, 2, 2, sw::kVec8>();
+        case 22: return tiled_config<R, 1, 2, 4, sw::kVec8>();
+        case 23: return tiled_config<R, 4, 2, 1, sw::kVec8>();
         default: return tiled_config<R, 1, 1, 4>();
         }
     } else {
         return v == 4 ? tiled_config<R, 4, 1, 4>()
              : v == 8 ? tiled_config<R, 8, 1, 1>()
-                      : tiled_config<R, 1, 1, 4>();
+             : v == 17 ? tiled_config<R, 4, 2, 1, sw::kVec8>()
+             : v == 18 ? tiled_config<R, 2, 2, 1, sw::kVec8>()
+                       : tiled_config<R, 1, 1, 4>();
     }
 }
 
@@ -895,7 +912,7 @@ void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
     const long long pbytes = (long long)P.nx * P.ny * (long long)sizeof(real_t);
     // buffer offsets are 32-bit: the planes one tile touches (at most
     // 64 + 2R) plus one row of overrun must stay below 2 GiB
-    if (pbytes * (64 + 2 * R + 1) >= (1ll << 31) || P.ny < sw::kVec)
+    if (pbytes * (64 + 2 * R + 1) >= (1ll << 31) || P.ny < sw::kVec16)
         return;
     sw::Tiling T{};
     T.ty = K.ty;

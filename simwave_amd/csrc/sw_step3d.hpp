@@ -1,10 +1,11 @@
-// Tiled 3D step kernel for gfx950 with 16-byte lanes.
+// Tiled 3D step kernel for gfx950 with 16- or 8-byte lanes.
 //
 // One step of the reference time loop (stencil, sources, boundaries; wave.c
 // sections 1-3) over the interior, fused in one launch.  Every lane owns V
-// consecutive points of the contiguous axis y (V = 4 floats or 2 doubles, one
-// 16-byte buffer access); a wavefront covers WR rows of 64/WR lanes, so a tile
-// row is TY = 64/WR * V points wide.  A work-group of NW wavefronts owns a tile
+// consecutive points of the contiguous axis y, moved by one buffer access
+// (16 bytes: V = 4 floats or 2 doubles; 8 bytes: 2 floats or 1 double); a
+// wavefront covers WR rows of 64/WR lanes, so a tile row is TY = 64/WR * V
+// points wide.  A work-group of NW wavefronts owns a tile
 // of TXR = NW*WR*RPT rows along x and marches it along z over zlen planes:
 //   * z neighbours: a per-lane register queue of 2R+1 vectors per row;
 //   * x/y neighbours: the current plane in LDS with an R-wide halo (rounded
@@ -33,14 +34,15 @@ namespace sw {
 
 #if SW_DIM == 3
 
-#if defined(SW_DOUBLE)
-constexpr int kVec = 2;
-typedef double vreal __attribute__((ext_vector_type(2)));
-#else
-constexpr int kVec = 4;
-typedef float vreal __attribute__((ext_vector_type(4)));
-#endif
+// A lane's V points as one vector register group.
+template <int V>
+struct LaneVec {
+    typedef real_t type __attribute__((ext_vector_type(V)));
+};
+constexpr int kVec16 = 16 / (int)sizeof(real_t);   // points per 16-byte lane
+constexpr int kVec8 = 8 / (int)sizeof(real_t);     // points per 8-byte lane
 typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
+typedef unsigned v2u32 __attribute__((ext_vector_type(2)));
 
 struct Tiling {
     int gk, gj, gz;     // tiles along y (ty wide), x (txr rows), z (zlen planes)
@@ -123,10 +125,16 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *base, unsigned long long
                                              0x00020000);
 }
 
-// 16-byte buffer accesses; the values are bit-cast, not converted.
-__device__ __forceinline__ vreal vload(rsrc_t r, unsigned voff, unsigned soff)
+// 16- or 8-byte buffer accesses; the values are bit-cast, not converted.
+template <int V>
+__device__ __forceinline__ typename LaneVec<V>::type vload(rsrc_t r, unsigned voff,
+                                                           unsigned soff)
 {
-    return __builtin_bit_cast(vreal, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    typedef typename LaneVec<V>::type vt;
+    if constexpr (sizeof(vt) == 16)
+        return __builtin_bit_cast(vt, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(vt, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
 // The store adds the plane offset into the VGPR offset and passes a zero
@@ -136,16 +144,21 @@ __device__ __forceinline__ vreal vload(rsrc_t r, unsigned voff, unsigned soff)
 // wait-state table exempts those).  On gfx950 an SGPR-offset store followed
 // at once by a write of its data registers stored the new values (seen as
 // byte offsets in the wavefield); with the zero SGPR offset the wait states
-// are there.
-__device__ __forceinline__ void vstore(vreal v, rsrc_t r, unsigned voff, unsigned soff)
+// are there.  8-byte stores take the same form.
+template <typename VT>
+__device__ __forceinline__ void vstore(VT v, rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), r, voff + soff, 0u, 0);
+    if constexpr (sizeof(VT) == 16)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), r, voff + soff,
+                                               0u, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u32, v), r, voff + soff, 0u,
+                                              0);
 }
 
 __device__ __forceinline__ void sstore(real_t v, rsrc_t r, unsigned voff, unsigned soff)
 {
 #if defined(SW_DOUBLE)
-    typedef unsigned v2u32 __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u32, v), r, voff, soff, 0);
 #else
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
@@ -220,9 +233,9 @@ struct StepArgs {
     Tiling T;
 };
 
-template <int R, int WR, int RPT, int NW>
+template <int R, int WR, int RPT, int NW, int VP>
 struct TileShape {
-    static constexpr int V = kVec;
+    static constexpr int V = VP;
     static constexpr int LW = 64 / WR;                 // lanes per row
     static constexpr int TY = LW * V;                  // tile width along y
     static constexpr int TXR = NW * WR * RPT;          // tile rows along x
@@ -243,11 +256,12 @@ struct TileShape {
 // non-float quotient at least 2^-49 from a rounding midpoint, and with t <= 0
 // no quotient in the subnormal range lands on a midpoint).  Otherwise the
 // fixed-divisor float quotients with their range vote.
-template <int R, int WR, int RPT, int NW, bool D64 = false>
+template <int R, int WR, int RPT, int NW, int VP, bool D64 = false>
 __global__ void __launch_bounds__(64 * NW)
 k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
 {
-    using S = TileShape<R, WR, RPT, NW>;
+    using S = TileShape<R, WR, RPT, NW, VP>;
+    typedef typename LaneVec<VP>::type vreal;
     constexpr int V = S::V, LW = S::LW, TY = S::TY, TXR = S::TXR, PADL = S::PADL;
     constexpr int HV = S::HV, LS = S::LS, H = S::H, SLOTS = S::SLOTS, HX = S::HX;
     constexpr int NYV = S::NYV, HY = S::HY;
@@ -404,7 +418,7 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
     for (int m = 0; m < 2 * R; m++)
 #pragma unroll
         for (int q = 0; q < RPT; q++)
-            qz[q][m + 1] = vload(r_uc, cb[q] + (unsigned)m * PB, 0u);
+            qz[q][m + 1] = vload<V>(r_uc, cb[q] + (unsigned)m * PB, 0u);
     // One register set per prefetched quantity: the centre plane and its
     // halos for plane i+1 are loaded right after plane i is written to LDS;
     // u_prev, C and the damping term for plane i+1 right after plane i's
@@ -416,25 +430,25 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
         const unsigned vc = (unsigned)(I + R - zb) * PB;
 #pragma unroll
         for (int q = 0; q < RPT; q++)
-            nc[q] = vload(r_uc, cb[q] + vc, 0u);
+            nc[q] = vload<V>(r_uc, cb[q] + vc, 0u);
 #pragma unroll
         for (int h = 0; h < HX; h++)
-            nhx[h] = vload(r_uc, xh_off[h], sp);
+            nhx[h] = vload<V>(r_uc, xh_off[h], sp);
 #pragma unroll
         for (int h = 0; h < HY; h++)
-            nhy[h] = vload(r_uc, yh_off[h], sp);
+            nhy[h] = vload<V>(r_uc, yh_off[h], sp);
     };
     auto load_update = [&](int I) {
         const unsigned sp = (unsigned)(I - zb) * PB;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
-            np[q] = vload(r_up, cb[q], sp);
-            nq[q] = vload(r_q, cb[q], sp);
+            np[q] = vload<V>(r_up, cb[q], sp);
+            nq[q] = vload<V>(r_q, cb[q], sp);
         }
         if ((dbits >> (I - z0)) & 1) {
 #pragma unroll
             for (int q = 0; q < RPT; q++)
-                ne[q] = vload(r_e, cb[q], sp);
+                ne[q] = vload<V>(r_e, cb[q], sp);
         }
     };
     // Clean-tile loads: always issued; the prefetch past the last plane
@@ -445,13 +459,13 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
         const unsigned vc = (unsigned)(I + R - zb) * PB;
 #pragma unroll
         for (int q = 0; q < RPT; q++)
-            nc[q] = vload(r_uc, live ? cb[q] + vc : kOOB, 0u);
+            nc[q] = vload<V>(r_uc, live ? cb[q] + vc : kOOB, 0u);
 #pragma unroll
         for (int h = 0; h < HX; h++)
-            nhx[h] = vload(r_uc, live ? xh_off[h] : kOOB, sp);
+            nhx[h] = vload<V>(r_uc, live ? xh_off[h] : kOOB, sp);
 #pragma unroll
         for (int h = 0; h < HY; h++)
-            nhy[h] = vload(r_uc, live ? yh_off[h] : kOOB, sp);
+            nhy[h] = vload<V>(r_uc, live ? yh_off[h] : kOOB, sp);
     };
     auto load_update_clean = [&](int I) {
         const bool live = I < z1;
@@ -459,9 +473,9 @@ k_step3d_vec(StepArgs A, const Params *__restrict__ Pd, int n)
         const unsigned sp = live ? (unsigned)(I - zb) * PB : 0u;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
-            np[q] = vload(r_up, live ? cb[q] : kOOB, sp);
-            nq[q] = vload(r_q, live ? cb[q] : kOOB, sp);
-            ne[q] = vload(r_e, damp ? cb[q] : kOOB, sp);
+            np[q] = vload<V>(r_up, live ? cb[q] : kOOB, sp);
+            nq[q] = vload<V>(r_q, live ? cb[q] : kOOB, sp);
+            ne[q] = vload<V>(r_e, damp ? cb[q] : kOOB, sp);
         }
     };
     // The scheduler barriers keep the prologue's loads in the loop's order

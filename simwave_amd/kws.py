@@ -8,7 +8,6 @@ few points around each position, for all positions at once; every weight is
 computed by the same numpy/scipy operations on the same operand types, so the
 values are identical.
 """
-import warnings
 
 import numpy as np
 from scipy.special import i0
@@ -38,8 +37,9 @@ def _kws(x, half_width):
     (tests/test_source.py:111-164 of the reference pins these float32 values
     bit for bit; a float64 normaliser changes the last bit).
     """
-    with warnings.catch_warnings():
-        warnings.simplefilter("ignore")
+    # NumPy's error state is per thread (serving builds shots on several
+    # threads at once; the warnings module's filters are process-wide)
+    with np.errstate(invalid="ignore"):
         root = np.sqrt(1 - (x / half_width) ** 2)
     b = get_kaiser_half_width(half_width)
     kaiser = i0(b * root) / x.dtype.type(i0(b))

@@ -98,6 +98,12 @@ CASES = {
                        tf=0.05),
     "3d_wide_so6": dict(dim=3, shape=(24, 26, 270), space_order=6, damping=30.0,
                         tf=0.05),
+    # odd interior widths at radius 6 (lanes of 2 floats straddling the
+    # row end), float32 and float64
+    "3d_wide_so12": dict(dim=3, shape=(26, 28, 263), space_order=12, damping=30.0,
+                         tf=0.04),
+    "3d_wide_f64_so12": dict(dim=3, shape=(24, 26, 241), space_order=12, damping=30.0,
+                             dtype=F64, tf=0.03),
     # deep grids: with 8-plane z chunks most tiles touch no boundary face and
     # no source box (the tiled kernel's branch-free march)
     "3d_deep_so8": dict(dim=3, shape=(60, 30, 301), space_order=8, damping=30.0,
@@ -165,11 +171,13 @@ def test_parity_generic_path(gpu, name):
 
 
 # Tile configurations of the 3D tiled kernel (SIMWAVE_HIP_TILE): radius 4
-# carries 0..16, other radii 0, 4 and 8; SIMWAVE_HIP_ORDER picks the tile
-# order.
+# carries 0..23 (17..23 with 8-byte lanes), other radii 0, 4, 8, 17 and 18;
+# SIMWAVE_HIP_ORDER picks the tile order.
+R4_TILES = list(range(24))
+OTHER_TILES = [0, 4, 8, 17, 18]
 TILED_R4 = ["3d_cd_so8", "3d_wide_so8", "3d_wide_f64_so8", "3d_wide_bc", "3d_wide_h15_bc"]
 TILED_OTHER = ["3d_cd_so2", "3d_cd_so20", "3d_bc_mixed", "3d_src_halo", "3d_rect",
-               "3d_wide_so6"]
+               "3d_wide_so6", "3d_wide_so12", "3d_wide_f64_so12"]
 
 
 def _tiled(name, tile, order):
@@ -188,7 +196,7 @@ DEEP_R4 = ["3d_deep_so8", "3d_deep_so8_whole", "3d_deep_none", "3d_deep_f64", "3
 
 
 @pytest.mark.parametrize("order", [0, 1])
-@pytest.mark.parametrize("tile", list(range(17)))
+@pytest.mark.parametrize("tile", R4_TILES)
 @pytest.mark.parametrize("name", DEEP_R4)
 def test_parity_clean_tiles_r4(gpu, name, tile, order):
     """Tiles away from every face and source (8-plane z chunks), every
@@ -201,7 +209,7 @@ def test_parity_clean_tiles_r4(gpu, name, tile, order):
 
 
 @pytest.mark.parametrize("order", [0, 1])
-@pytest.mark.parametrize("tile", [0, 4, 8])
+@pytest.mark.parametrize("tile", OTHER_TILES)
 def test_parity_clean_tiles_r2(gpu, tile, order):
     os.environ["SIMWAVE_HIP_ZLEN"] = "6"
     try:
@@ -211,7 +219,7 @@ def test_parity_clean_tiles_r2(gpu, tile, order):
 
 
 @pytest.mark.parametrize("order", [0, 1])
-@pytest.mark.parametrize("tile", list(range(17)))
+@pytest.mark.parametrize("tile", R4_TILES)
 @pytest.mark.parametrize("name", TILED_R4)
 def test_parity_tiled_configs_r4(gpu, name, tile, order):
     """Every tile configuration of the radius-4 tiled 3D kernel."""
@@ -219,7 +227,7 @@ def test_parity_tiled_configs_r4(gpu, name, tile, order):
 
 
 @pytest.mark.parametrize("order", [0, 1])
-@pytest.mark.parametrize("tile", [0, 4, 8])
+@pytest.mark.parametrize("tile", OTHER_TILES)
 @pytest.mark.parametrize("name", TILED_OTHER)
 def test_parity_tiled_configs(gpu, name, tile, order):
     """The tile configurations the other radii carry."""
