@@ -734,23 +734,30 @@ void set_lap_div(Prepared &pr)
 // Tile configurations of k_step3d_vec as (rows per wavefront WR, rows per
 // thread RPT, wavefronts per block NW, and 16- or 8-byte lanes).  Radius 4
 // (space order 8) carries every configuration; other radii carry 0, 4, 8,
-// 17 and 18.  SIMWAVE_HIP_TILE picks one; otherwise default_tile(R).
-// Measured per launch on one MI355X (float32, 472^3): radius 4 is fastest
-// with 4 rows x 128 points of 8-byte lanes, 4 rows per lane, and 6-plane z
-// chunks (511 us; 521 us with 8 planes; 8 rows x 64 points of 8-byte lanes
-// 515 us; of 16-byte lanes, 2 rows per lane, 532 us).  The 8-byte lanes
-// halve the registers of each z queue entry, so a lane can hold 4 adjacent
-// rows and take more x neighbours from registers; the shapes with 3
-// wavefronts per SIMD (133-137 VGPRs) came to 541-578 us.  Radii 5-10: 8
-// rows x 32 points of 16-byte lanes and 4-plane chunks (10-13% ahead of
-// 4-wavefront tiles at 312^3-376^3; the 8-byte shapes 1-10% behind at 320^3
-// so12 and so16).  Float64: 8 rows x 32 points of 16-byte lanes, 8-plane
-// chunks at radius 4 (472^3: 881 us; the 8-byte shapes 894-1097 us), 12-plane
-// chunks elsewhere.
+// 17 and 18.  SIMWAVE_HIP_TILE picks one; otherwise default_tile().
+// Measured per launch on one MI355X, float32, radius 4, 6-plane z chunks:
+//  * spacings whose Laplacian quotients are float64 products (div64):
+//    8 rows x 64 points of 8-byte lanes with 4 rows per lane are fastest,
+//    1-3% ahead of the same tile of 16-byte lanes with 2 rows per lane
+//    (472^3: 515 against 532 us; interior 340^3 245 against 248, 380^3 318
+//    against 322, 420^3 411 against 420; 300^3 even).  The 8-byte lanes halve the registers of a z
+//    queue entry, so a lane holds 4 adjacent rows and takes more of its x
+//    neighbours from registers.  The 4 x 128 shape of 8-byte lanes was
+//    fastest at 472^3 alone (511 us) and 2-8% behind elsewhere; shapes with
+//    3 wavefronts per SIMD (133-137 VGPRs) came to 541-578 us at 472^3.
+//  * other spacings (fixed-divisor quotients): the 16-byte 8 x 64 tile, the
+//    8-byte one within 2% either way (300^3-380^3 interior), 5% behind at
+//    260^3.
+// Radii 5-10: 8 rows x 32 points of 16-byte lanes and 4-plane chunks
+// (10-13% ahead of 4-wavefront tiles at 312^3-376^3; the 8-byte shapes
+// 1-10% behind at 320^3 so12 and so16).  Float64: 8 rows x 32 points of
+// 16-byte lanes, 8-plane chunks at radius 4 (472^3: 881 us; the 8-byte
+// shapes 894-1097 us), 12-plane chunks elsewhere.
 constexpr int kTileShapes = 24;
-constexpr int default_tile(int R)
+constexpr int default_tile(int R, bool div64)
 {
-    return R == 4 ? (sizeof(real_t) == 4 ? 20 : 10) : R < 4 ? 0 : (sizeof(real_t) == 4 ? 8 : 4);
+    return R == 4 ? (sizeof(real_t) == 4 && div64 ? 17 : 10)
+         : R < 4 ? 0 : (sizeof(real_t) == 4 ? 8 : 4);
 }
 constexpr int default_tile_zlen(int R)
 {
@@ -763,10 +770,11 @@ constexpr int default_tile_zlen(int R)
 // neighbour reads through the caches lose once there are plenty at radius 4.
 // Measured per launch (float32, cubes): radius 1-3 at every size up to
 // 466^3 (1.2-1.9x faster than any tile configuration); radius 4 up to
-// ~27 M interior points (228 x 324 x 324, 22 M: 138 against 149 us; 320^3,
-// 33 M: tiled 196 against 207 us; 384^3, 57 M: 327 against 347 us; with
-// the tiled kernel's 6-plane chunks and Dirichlet faces in its clean
-// march); radius 5+ never (its register queue is 2R+1 deep).
+// ~30 M interior points (interior 260^3: 126 against 144 us; 300^3, 27 M:
+// 172 against 187 us, 198 us with float64-product quotients; 320^3, 33 M:
+// tiled 192 against 209 us; 384^3, 57 M: 327 against 347 us; with the
+// tiled kernel's 6-plane chunks and Dirichlet faces in its clean march);
+// radius 5+ never (its register queue is 2R+1 deep).
 // Float64: the tiled kernel (z-marching 26% slower at 22 M, radius 4).
 // Variable density: float32 at every radius and size measured, 1.2-1.8x
 // faster than the LDS-tiled k_step3d (176^3-448^3, orders 2-16); float64 at
@@ -783,7 +791,7 @@ bool prefer_zmarch(const Prepared &pr)
     const long long interior = (long long)(P.nz - 2 * R) * (P.nx - 2 * R) * (P.ny - 2 * R);
     if (R <= 3)
         return true;
-    return R == 4 && interior < 27ll * 1000 * 1000;
+    return R == 4 && interior < 30ll * 1000 * 1000;
 }
 
 template <int R, int WR, int RPT, int NW, int V, bool D64>
@@ -905,8 +913,13 @@ void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
 {
     const Params &P = pr.P;
     const int R = pr.r;
-    int shape = env_int("SIMWAVE_HIP_TILE", default_tile(R));
-    shape = shape >= 0 && shape < kTileShapes ? shape : default_tile(R);
+    // float32 Laplacian quotients as float64 products with RN(1/h^2)
+    // where that gives the IEEE quotient for every numerator
+    const bool div64 = sizeof(real_t) == 4 && !env_flag("SIMWAVE_HIP_IEEE_DIV") &&
+                       !env_flag("SIMWAVE_HIP_NO_DIV64") && f64_quotient_exact(P.dz2) &&
+                       f64_quotient_exact(P.dx2) && f64_quotient_exact(P.dy2);
+    int shape = env_int("SIMWAVE_HIP_TILE", default_tile(R, div64));
+    shape = shape >= 0 && shape < kTileShapes ? shape : default_tile(R, div64);
     const TiledKernel K = tiled_kernel(R, shape);
     const int Iy = P.ny - 2 * R, Ix = P.nx - 2 * R, Iz = P.nz - 2 * R;
     const long long pbytes = (long long)P.nx * P.ny * (long long)sizeof(real_t);
@@ -966,9 +979,7 @@ void setup_tiling(Prepared &pr, DeviceArena &arena, hipStream_t st)
     }
     pr.fastdiv = !env_flag("SIMWAVE_HIP_IEEE_DIV") && make_fixed_div(P.dz2, pr.dvz) &&
                  make_fixed_div(P.dx2, pr.dvx) && make_fixed_div(P.dy2, pr.dvy);
-    pr.div64 = sizeof(real_t) == 4 && !env_flag("SIMWAVE_HIP_IEEE_DIV") &&
-               !env_flag("SIMWAVE_HIP_NO_DIV64") && f64_quotient_exact(P.dz2) &&
-               f64_quotient_exact(P.dx2) && f64_quotient_exact(P.dy2);
+    pr.div64 = div64;
     pr.tiled_launch = pr.div64 ? K.launch64 : K.launch;
     pr.tiling = T;
     pr.tiled_blocks = (int)blocks;
