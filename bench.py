@@ -467,7 +467,6 @@ def run_fwi(args):
 def graph_replay_effect(reps=3):
     """A launch-bound shot (2D 128^2, order 2, 960 steps) run through a plan
     in 48-step chunks, with direct launches and with graph replay."""
-    import torch
     import simwave_amd as sw
     from simwave_amd.device import DeviceForward, FLAG_GRAPH
     vel = np.full((128, 128), 2000.0, dtype=np.float32)
@@ -482,7 +481,8 @@ def graph_replay_effect(reps=3):
                        receivers=sw.Receiver(sm, coordinates=[(10.0, x) for x in
                                                               np.linspace(0, 1270, 128)]),
                        wavelet=sw.RickerWavelet(15.0, tm))
-    stream = torch.cuda.Stream()
+    from simwave_amd import hipmem
+    stream = hipmem.Stream(hipmem.default_library())
     out = {"shot": "2D 128^2 so2, %d steps, 48-step chunks" % tm.timesteps}
     for name, flags in (("direct_ms", 0), ("graph_ms", FLAG_GRAPH)):
         best = None

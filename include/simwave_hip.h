@@ -23,6 +23,9 @@
  *     edge-extended models and damping mask of a device-resident shot;
  *   - simwave_hip_fixed_div_<variant>, simwave_hip_last_error,
  *     simwave_hip_last_timings, simwave_hip_variant, simwave_hip_trim_pool;
+ *   - simwave_hip_{device,host}_{alloc,free}, simwave_hip_copy,
+ *     simwave_hip_zero, simwave_hip_stream_*, simwave_hip_event_*: memory,
+ *     streams and events for device-resident callers;
  *   - through libsimwave_hip_common.so, which every variant library links:
  *     simwave_hip_overlap_{acquire,release,holders}.
  *
@@ -1459,6 +1462,36 @@ const char *simwave_hip_variant(void);
  * allocations; up to 8 GiB is kept between runs) back to the device, after
  * waiting for the device, so that a free-memory query sees it.  0, or -1. */
 int simwave_hip_trim_pool(void);
+
+/* No reference counterpart (the reference's forward takes host arrays
+ * only, wave.c:23-36).  Device memory, pinned host memory, copies, zero
+ * fills, streams and events on the current device, for callers that keep
+ * a shot resident through simwave_hip_forward_device_<variant> or the plan
+ * entries and hold no other GPU runtime (the Python device path and serving
+ * loop use these).  Streams are non-blocking with respect to the null
+ * stream, at the default priority.  Each returns 0, or -1 with
+ * simwave_hip_last_error() set. */
+/* Stream-ordered device memory from the current device's default pool:
+ * allocated for work enqueued on `stream` after the call; a free returns it
+ * once the work enqueued on `stream` before the call is done (work on other
+ * streams that reads it must be complete).  Neither waits on the host. */
+int simwave_hip_device_alloc(unsigned long long bytes, void *stream, void **ptr);
+int simwave_hip_device_free(void *ptr, void *stream);
+int simwave_hip_host_alloc(unsigned long long bytes, void **ptr);
+int simwave_hip_host_free(void *ptr);
+/* Stream-ordered copy of `bytes` between device, pinned or pageable host
+ * memory (the direction follows from the pointers). */
+int simwave_hip_copy(void *dst, const void *src, unsigned long long bytes, void *stream);
+/* Zeroes `bytes` (a multiple of 4) of device memory with a fill kernel on
+ * `stream`. */
+int simwave_hip_zero(void *dst, unsigned long long bytes, void *stream);
+int simwave_hip_stream_create(void **stream);
+int simwave_hip_stream_destroy(void *stream);
+int simwave_hip_stream_synchronize(void *stream);
+int simwave_hip_event_create(void **event);
+int simwave_hip_event_record(void *event, void *stream);
+int simwave_hip_event_synchronize(void *event);
+int simwave_hip_event_destroy(void *event);
 
 /* ---- libsimwave_hip_common.so (linked by every variant library; one copy
  * per process).  No reference counterpart.  The receiver-overlap lease:

@@ -83,7 +83,6 @@ def _run(dev, rec_adjoint, checkpoint_every, want_wavelet, stream=None, image=Fa
     # the zeroed outputs above are queued on the current stream, the shot's
     # uploads on the stream it was built on
     stream.wait_stream(torch.cuda.current_stream(dev.device))
-    stream.wait_event(dev._ready)
     secs = fn(*dev._args, ctypes.c_void_p(rec.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
               ctypes.c_void_p(gw.data_ptr() if gw is not None else 0),
               ctypes.c_size_t(int(checkpoint_every)), ctypes.c_void_p(stream.cuda_stream),
@@ -119,7 +118,7 @@ def gradient(solver, rec_adjoint, checkpoint_every=None, wavelet_gradient=False,
     ([timesteps, sources]).
     """
     _check(solver)
-    dev = DeviceForward(solver, device=device, velocity=velocity)
+    dev = DeviceForward(solver, device=device, velocity=velocity, buffers="torch")
     grad, gw = _run(dev, rec_adjoint, checkpoint_every, wavelet_gradient)
     out = {"shot_record": dev.tensors["shot_record"].cpu().numpy(),
            "grad_velocity": grad.cpu().numpy()}
@@ -134,7 +133,7 @@ def image(solver, rec_adjoint, checkpoint_every=None, velocity=None, device="cud
     (reverse-time migration of the data `rec_adjoint`), float64 over the
     extended grid, zero outside the interior."""
     _check(solver)
-    dev = DeviceForward(solver, device=device, velocity=velocity)
+    dev = DeviceForward(solver, device=device, velocity=velocity, buffers="torch")
     img, _ = _run(dev, rec_adjoint, checkpoint_every, False, image=True)
     return img.cpu().numpy()
 
@@ -143,7 +142,7 @@ def misfit_gradient(solver, observed, checkpoint_every=None, velocity=None, devi
     """1/2 ||d - observed||^2 and its gradient with respect to the extended
     velocity (float64, extended shape)."""
     _check(solver)
-    dev = DeviceForward(solver, device=device, velocity=velocity)
+    dev = DeviceForward(solver, device=device, velocity=velocity, buffers="torch")
     misfit, grad = shot_misfit_gradient(dev, observed, checkpoint_every)
     return misfit, grad.cpu().numpy()
 
@@ -181,7 +180,7 @@ def _autograd():
         def forward(ctx, velocity_ext, solver, checkpoint_every=None):
             _check(solver)
             vel = velocity_ext.detach().contiguous()
-            dev = DeviceForward(solver, device=vel.device, velocity=vel)
+            dev = DeviceForward(solver, device=vel.device, velocity=vel, buffers="torch")
             dev.run()
             ctx.solver = solver
             ctx.checkpoint_every = checkpoint_every
@@ -191,7 +190,7 @@ def _autograd():
         @staticmethod
         def backward(ctx, grad_rec):
             (vel,) = ctx.saved_tensors
-            dev = DeviceForward(ctx.solver, device=vel.device, velocity=vel)
+            dev = DeviceForward(ctx.solver, device=vel.device, velocity=vel, buffers="torch")
             grad, _ = _run(dev, grad_rec.detach(), ctx.checkpoint_every, False)
             return grad.to(vel.dtype), None, None
 

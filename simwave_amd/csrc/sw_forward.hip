@@ -3033,6 +3033,175 @@ SW_EXPORT const char *simwave_hip_last_error(void)
     return g_error.c_str();
 }
 
+// Device memory, pinned host memory, copies, zero fills, streams and
+// events on the current device, so that a caller holding no other GPU
+// runtime can keep a shot resident and drive it on streams of its own.
+// Device memory is stream-ordered (the default pool, which keeps freed
+// memory for reuse up to kPoolKeepBytes): a free waits for nothing on the
+// host, where hipFree would wait for the whole device, so shots that end
+// while others run do not stall.  Streams are non-blocking with respect to
+// the null stream, at the default priority.  Each returns 0, or -1 with
+// simwave_hip_last_error() set.
+SW_EXPORT int simwave_hip_device_alloc(unsigned long long bytes, void *stream, void **ptr)
+{
+    const double r = guarded([&] {
+        if (!ptr)
+            fail("device_alloc: NULL result pointer");
+        *ptr = nullptr;
+        keep_pool_memory();
+        hip_check(hipMallocAsync(ptr, bytes ? bytes : 1, (hipStream_t)stream),
+                  "hipMallocAsync");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+// The memory returns to the pool once the work enqueued on `stream` before
+// the call is done; work on other streams that reads it must be complete.
+SW_EXPORT int simwave_hip_device_free(void *ptr, void *stream)
+{
+    const double r = guarded([&] {
+        if (ptr)
+            hip_check(hipFreeAsync(ptr, (hipStream_t)stream), "hipFreeAsync");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_host_alloc(unsigned long long bytes, void **ptr)
+{
+    const double r = guarded([&] {
+        if (!ptr)
+            fail("host_alloc: NULL result pointer");
+        *ptr = nullptr;
+        hip_check(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault),
+                  "hipHostMalloc");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_host_free(void *ptr)
+{
+    const double r = guarded([&] {
+        if (ptr)
+            hip_check(hipHostFree(ptr), "hipHostFree");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+// Stream-ordered copy of `bytes` between any two of device, pinned and
+// pageable host memory (the direction follows from the pointers).
+SW_EXPORT int simwave_hip_copy(void *dst, const void *src, unsigned long long bytes,
+                               void *stream)
+{
+    const double r = guarded([&] {
+        if (bytes && (!dst || !src))
+            fail("copy: NULL pointer");
+        if (bytes)
+            hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, (hipStream_t)stream),
+                      "hipMemcpyAsync");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+// Zeroes `bytes` (a multiple of 4) of device memory with a fill kernel on
+// `stream` (a kernel, not a memset node, so that it also behaves under
+// graph capture).
+SW_EXPORT int simwave_hip_zero(void *dst, unsigned long long bytes, void *stream)
+{
+    const double r = guarded([&] {
+        if (bytes % 4)
+            fail("zero: %llu bytes is not a multiple of 4", bytes);
+        if (bytes && !dst)
+            fail("zero: NULL pointer");
+        const long long n = (long long)(bytes / 4);
+        if (n) {
+            const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+            hipLaunchKernelGGL(sw::k_zero_u32, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                               (unsigned *)dst, n);
+            hip_check(hipGetLastError(), "zero launch");
+        }
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_stream_create(void **stream)
+{
+    const double r = guarded([&] {
+        if (!stream)
+            fail("stream_create: NULL result pointer");
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        *stream = s;
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_stream_destroy(void *stream)
+{
+    const double r = guarded([&] {
+        if (stream)
+            hip_check(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_stream_synchronize(void *stream)
+{
+    const double r = guarded([&] {
+        hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_event_create(void **event)
+{
+    const double r = guarded([&] {
+        if (!event)
+            fail("event_create: NULL result pointer");
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        *event = e;
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_event_record(void *event, void *stream)
+{
+    const double r = guarded([&] {
+        hip_check(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_event_synchronize(void *event)
+{
+    const double r = guarded([&] {
+        hip_check(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
+SW_EXPORT int simwave_hip_event_destroy(void *event)
+{
+    const double r = guarded([&] {
+        if (event)
+            hip_check(hipEventDestroy((hipEvent_t)event), "hipEventDestroy");
+        return 0.0;
+    });
+    return r < 0 ? -1 : 0;
+}
+
 // Returns the freed memory the current device's default pool keeps for
 // reuse to the device (the library's stream-ordered allocations come from
 // that pool), so that a free-memory query afterwards sees it.  0, or -1.

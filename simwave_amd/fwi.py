@@ -22,7 +22,7 @@ def observed_data(setups, device="cuda"):
     dictionaries), one device tensor [timesteps, receivers] each."""
     out = []
     for parts in setups:
-        dev = DeviceForward(Solver(**parts), device=device)
+        dev = DeviceForward(Solver(**parts), device=device, buffers="torch")
         dev.run(flags=0)
         out.append(dev.tensors["shot_record"].clone())
         del dev
@@ -35,7 +35,7 @@ def modelled_shots(setups, device="cuda"):
     shots = []
     vel = None
     for parts in setups:
-        dev = DeviceForward(Solver(**parts), device=device, velocity=vel)
+        dev = DeviceForward(Solver(**parts), device=device, velocity=vel, buffers="torch")
         adjoint._check(dev.solver)
         vel = dev.tensors["velocity_model"]
         shots.append(dev)

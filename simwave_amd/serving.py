@@ -25,7 +25,9 @@ while others run.  The server
 * records per-request latency (arrival to result on the host).
 
 A batched shot runs the same per-point operations on the same inputs as
-alone, so every result is bit-identical to ``Solver.forward()``.
+alone, so every result is bit-identical to ``Solver.forward()``.  Device
+memory, pinned staging, streams and events are the HIP library's own
+(``simwave_amd.hipmem``); the server needs no other GPU runtime.
 """
 import collections
 import threading
@@ -159,6 +161,7 @@ class ShotServer:
         self._used = 0.0
         self._closed = False
         self._workers = []
+        self._idle_streams = []       # worker streams that finished a shot
         self.admissions = []          # submission numbers (0, 1, ...) in admission order
         self._submitted = 0
         self.batch = bool(batch)
@@ -205,13 +208,13 @@ class ShotServer:
         self.close(wait=True)
 
     # ------------------------------------------------------------ internals
-    def _device_shot(self, solver):
+    def _device_shot(self, solver, stream=None):
         from .device import DeviceForward
-        return DeviceForward(solver, device=self.device)
+        return DeviceForward(solver, device=self.device, stream=stream)
 
     def _new_stream(self):
-        import torch
-        return torch.cuda.Stream(device=self.device)
+        from . import hipmem
+        return hipmem.Stream(hipmem.default_library())
 
     def _schedule(self):
         while True:
@@ -257,16 +260,17 @@ class ShotServer:
 
     def _work(self, req, need, active_now):
         shot = None
+        stream = None
         try:
-            stream = self._stream_factory()
-            try:
-                import torch
-                ctx = (torch.cuda.stream(stream) if isinstance(stream, torch.cuda.Stream)
-                       else _NullContext())     # test doubles: no torch stream
-            except ImportError:
-                ctx = _NullContext()
+            with self._cv:
+                stream = self._idle_streams.pop() if self._idle_streams else None
+            if stream is None:
+                stream = self._stream_factory()
             t0 = time.perf_counter()
-            with ctx:
+            if self._shot_factory == self._device_shot:
+                # built on the worker's stream, which its memory is ordered on
+                shot = self._shot_factory(req.solver, stream)
+            else:
                 shot = self._shot_factory(req.solver)
             tb = time.perf_counter()
             shot.plan(stream, flags=self.plan_flags)
@@ -301,6 +305,8 @@ class ShotServer:
         finally:
             shot = None
             with self._cv:
+                if stream is not None:
+                    self._idle_streams.append(stream)
                 self._used -= need
                 self._active -= 1
                 self._cv.notify_all()
@@ -379,11 +385,3 @@ class _BatchRunner:
                         del self._shots[shot]
                         slot[1] = err
                         slot[0].set()
-
-
-class _NullContext:
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        return False

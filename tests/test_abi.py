@@ -37,7 +37,13 @@ def test_header_declares_every_variant():
     for common in ("simwave_hip_last_error", "simwave_hip_last_timings",
                    "simwave_hip_variant", "simwave_hip_trim_pool",
                    "simwave_hip_overlap_acquire", "simwave_hip_overlap_release",
-                   "simwave_hip_overlap_holders"):
+                   "simwave_hip_overlap_holders", "simwave_hip_device_alloc",
+                   "simwave_hip_device_free", "simwave_hip_host_alloc",
+                   "simwave_hip_host_free", "simwave_hip_copy", "simwave_hip_zero",
+                   "simwave_hip_stream_create", "simwave_hip_stream_destroy",
+                   "simwave_hip_stream_synchronize", "simwave_hip_event_create",
+                   "simwave_hip_event_record", "simwave_hip_event_synchronize",
+                   "simwave_hip_event_destroy"):
         assert common in names
 
 
@@ -167,3 +173,31 @@ def test_receiver_overlap_lease_is_process_wide():
     for line in calls:
         assert "receiver overlap (flag 4) is already in use" in line, line
     assert "DONE" in res.stdout
+
+
+def test_memory_entry_points_check_their_arguments():
+    """The memory, copy and zero entry points refuse bad arguments with an
+    error return and a message, before any device work."""
+    lib = ctypes.CDLL(sw.build.library_path(3, "cd", "f32"))
+    lib.simwave_hip_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    lib.simwave_hip_zero.argtypes = [vp, ctypes.c_ulonglong, vp]
+    lib.simwave_hip_copy.argtypes = [vp, vp, ctypes.c_ulonglong, vp]
+    lib.simwave_hip_device_alloc.argtypes = [ctypes.c_ulonglong, vp, vp]
+    lib.simwave_hip_device_free.argtypes = [vp, vp]
+    lib.simwave_hip_host_alloc.argtypes = [ctypes.c_ulonglong, vp]
+    assert lib.simwave_hip_zero(None, 6, None) == -1
+    assert "multiple of 4" in lib.simwave_hip_last_error().decode()
+    assert lib.simwave_hip_zero(None, 8, None) == -1
+    assert "NULL pointer" in lib.simwave_hip_last_error().decode()
+    assert lib.simwave_hip_copy(None, None, 8, None) == -1
+    assert "NULL pointer" in lib.simwave_hip_last_error().decode()
+    assert lib.simwave_hip_device_alloc(16, None, None) == -1
+    assert "NULL result pointer" in lib.simwave_hip_last_error().decode()
+    assert lib.simwave_hip_host_alloc(16, None) == -1
+    assert "NULL result pointer" in lib.simwave_hip_last_error().decode()
+    # zero-sized work and NULL frees are no-ops
+    assert lib.simwave_hip_zero(None, 0, None) == 0
+    assert lib.simwave_hip_copy(None, None, 0, None) == 0
+    assert lib.simwave_hip_device_free(None, None) == 0
+    assert lib.simwave_hip_host_free(None) == 0

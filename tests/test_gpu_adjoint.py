@@ -170,7 +170,7 @@ def test_default_keeps_every_level_when_it_fits():
     recomputation) and gives the checkpointed result."""
     _, solver, kw = cases.adjoint_shot("3d_so8")
     T = kw["end_timestep"]
-    dev = DeviceForward(solver)
+    dev = DeviceForward(solver, buffers="torch")
     assert adjoint.checkpoint_interval(dev) == T
     rr = np.random.default_rng(10).standard_normal((T, kw["num_receivers"]))
     a = adjoint.gradient(solver, rr, wavelet_gradient=True)
@@ -214,7 +214,7 @@ def test_timed_reverse_updates_keep_the_gradient(name):
     _, solver, kw = cases.adjoint_shot(name, dtype=F32)
     rng = np.random.default_rng(5)
     obs = rng.standard_normal((kw["end_timestep"], kw["num_receivers"])).astype(F32)
-    dev = DeviceForward(solver)
+    dev = DeviceForward(solver, buffers="torch")
     m0, g0 = adjoint.shot_misfit_gradient(dev, obs, checkpoint_every=5)
     m1, g1 = adjoint.shot_misfit_gradient(dev, obs, checkpoint_every=5, time_update=True)
     t = dev.timings()

@@ -12,6 +12,7 @@ import pytest
 
 import cases
 import simwave_amd as sw
+from simwave_amd import hipmem
 from simwave_amd.device import DeviceForward
 
 pytestmark = pytest.mark.gpu
@@ -44,12 +45,11 @@ def _alone(params):
 
 
 def test_batched_runs_equal_shots_alone(gpu):
-    import torch
     alone = []
     for params, _ in MIX:
         print("alone", params, flush=True)
         alone.append(_alone(params))
-    stream = torch.cuda.Stream()
+    stream = hipmem.Stream(hipmem.default_library())
     shots = []
     for params, want in MIX:
         dev = DeviceForward(sw.Solver(**cases.make_case(**params)()))
@@ -81,8 +81,7 @@ def test_batched_runs_equal_shots_alone(gpu):
 
 
 def test_batched_run_rejects_mixed_libraries(gpu):
-    import torch
-    stream = torch.cuda.Stream()
+    stream = hipmem.Stream(hipmem.default_library())
     a = DeviceForward(sw.Solver(**cases.make_case(dim=2, n=40)()))
     b = DeviceForward(sw.Solver(**cases.make_case(dim=3, n=24, tf=0.05)()))
     a.plan(stream)

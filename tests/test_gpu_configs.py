@@ -239,7 +239,7 @@ def test_fwi_gradient_finite_differences(fwi_setup):
     shot = 5
     obs = fwi.observed_data([_shot(true, shot, F64, dt)])[0]
     solver = sw.Solver(**_shot(init, shot, F64, dt))
-    dev = DeviceForward(solver)
+    dev = DeviceForward(solver, buffers="torch")
     vel = dev.tensors["velocity_model"].clone()
     m0, g = adjoint.shot_misfit_gradient(dev, obs, checkpoint_every=20)
     g = g.cpu().numpy()

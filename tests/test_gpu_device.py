@@ -6,6 +6,7 @@ import pytest
 
 import cases
 import simwave_amd as sw
+from simwave_amd import hipmem
 from simwave_amd.device import DeviceForward
 from simwave_amd.middleware import Middleware
 
@@ -27,9 +28,10 @@ def test_device_arrays_equal_host_arrays(name):
     solver = sw.Solver(**cases.make_case(**CASES[name])())
     host = Middleware.normalize(**solver.forward_arguments())
     dev = DeviceForward(solver)
+    assert dev.buffers == "library"
     for key, t in dev.tensors.items():
         ref = host[key]
-        got = t.cpu().numpy()
+        got = t.numpy()
         assert got.shape == ref.shape and got.dtype == ref.dtype, key
         assert np.array_equal(got, ref), key
 
@@ -37,17 +39,17 @@ def test_device_arrays_equal_host_arrays(name):
 @pytest.mark.parametrize("name", ["3d_cd", "2d_vd_f64"])
 def test_shot_built_on_one_stream_runs_on_another(name):
     """The shot's uploads and gathers are queued on the stream it is built
-    on; a plan, its chunks and the results fetched on another stream see
-    them (the shot's ready event), and give the same arrays as one stream."""
-    import torch
+    on, which the constructor synchronises; a plan, its chunks and the
+    results fetched on another of the library's streams see them and give
+    the same arrays as the shot's own stream."""
     print("cross-stream shot", name, flush=True)
     solver = sw.Solver(**cases.make_case(**CASES[name])())
     ref = DeviceForward(solver)
     ref.run()
     u_ref, rec_ref = ref.results()
-    build, other = torch.cuda.Stream(), torch.cuda.Stream()
-    with torch.cuda.stream(build):
-        dev = DeviceForward(solver)
+    lib = hipmem.default_library()
+    build, other = hipmem.Stream(lib), hipmem.Stream(lib)
+    dev = DeviceForward(solver, stream=build)
     dev.plan(other)
     left = dev.steps
     while left > 0:

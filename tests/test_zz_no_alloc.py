@@ -12,6 +12,7 @@ import pytest
 
 import cases
 import simwave_amd as sw
+from simwave_amd import hipmem
 from simwave_amd.device import DeviceForward
 from simwave_amd.middleware import _last_error
 
@@ -35,22 +36,19 @@ PAIRS = {
 
 
 def _call(dev, noalloc, flags=0):
-    import torch
     name = ("simwave_hip_forward_device_noalloc_" if noalloc
             else "simwave_hip_forward_device_") + dev.variant
     fn = getattr(dev._lib, name)
     fn.restype = ctypes.c_double
     fn.argtypes = list(dev._argtypes) + [ctypes.c_void_p, ctypes.c_uint]
-    dev.reset()
-    dev.tensors["shot_record"].zero_()
-    stream = torch.cuda.current_stream()
-    stream.wait_event(dev._ready)
-    secs = fn(*dev._args, ctypes.c_void_p(stream.cuda_stream), ctypes.c_uint(flags))
+    stream = dev._stream(None)
+    dev.reset(stream)
+    dev.tensors["shot_record"].zero_(stream)
+    secs = fn(*dev._args, ctypes.c_void_p(hipmem.stream_handle(stream)), ctypes.c_uint(flags))
     if secs < 0:
         raise RuntimeError(_last_error(dev._lib))
-    torch.cuda.synchronize()
-    return (dev.tensors["u_full"].cpu().numpy().copy(),
-            dev.tensors["shot_record"].cpu().numpy().copy())
+    dev.synchronize(stream)
+    return dev.tensors["u_full"].numpy(), dev.tensors["shot_record"].numpy()
 
 
 @pytest.mark.parametrize("generic", [0, 2], ids=["fused", "generic"])
