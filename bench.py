@@ -527,7 +527,8 @@ def run_serving(args):
     gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
     torch.cuda.synchronize()
     with ShotServer(hbm_budget=args.hbm_budget * 1e9, max_active=args.max_active,
-                    chunk_steps=args.chunk_steps, plan_flags=args.plan_flags) as srv:
+                    chunk_steps=args.chunk_steps, plan_flags=args.plan_flags,
+                    build_ahead=args.build_ahead) as srv:
         start = time.perf_counter()
         futs = []
         due = start
@@ -579,6 +580,7 @@ def run_serving(args):
     config = {"workload": "serving", "requests": len(reqs), "seed": args.seed,
               "arrival_rate_hz": args.rate, "hbm_budget_gb": args.hbm_budget,
               "max_active": args.max_active, "chunk_steps": args.chunk_steps,
+              "build_ahead": args.build_ahead,
               "plan_flags": args.plan_flags, "grids_2d": list(n2), "grids_3d": list(n3),
               "tf_scale": args.tf_scale,
               "description": workloads.serving_requests.__doc__.split("\n")[0]}
@@ -640,6 +642,8 @@ def main():
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving budget, GB")
     ap.add_argument("--max-active", type=int, default=32)
     ap.add_argument("--chunk-steps", type=int, default=48)
+    ap.add_argument("--build-ahead", type=int, default=4,
+                    help="serving: requests built ahead of admission on a thread pool")
     ap.add_argument("--plan-flags", type=int, default=None,
                     help="serving: plan flags (default: the server's, graph replay)")
     ap.add_argument("--seed", type=int, default=0)

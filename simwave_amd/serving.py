@@ -32,7 +32,7 @@ memory, pinned staging, streams and events are the HIP library's own
 import collections
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 
 import numpy as np
 
@@ -109,7 +109,7 @@ class ServeStats:
 
 
 class _Request:
-    __slots__ = ("solver", "arrival", "future", "admitted", "keep_result", "seq")
+    __slots__ = ("solver", "arrival", "future", "admitted", "keep_result", "seq", "build")
 
     def __init__(self, solver, arrival, seq=0):
         self.seq = seq
@@ -118,6 +118,11 @@ class _Request:
         self.future = Future()
         self.admitted = None
         self.keep_result = True
+        self.build = None       # Future of the Solver a callable request builds
+
+    def needs_build(self):
+        s = self.solver
+        return callable(s) and not hasattr(s, "space_model") and not hasattr(s, "device_bytes")
 
 
 class ShotServer:
@@ -141,10 +146,17 @@ class ShotServer:
         ``solver -> shot`` with the ``DeviceForward`` interface (``steps``,
         ``updates_per_shot``, ``plan``, ``run_steps``, ``results``,
         ``release_plan``); replaceable for tests, as is ``stream_factory``.
+    build_ahead : int
+        Requests given as callables are built (their Solver: models and
+        frontend arrays, on the host) by a pool of this many threads, for
+        the first `build_ahead` waiting requests at a time, so that building
+        overlaps the admission and runs of earlier shots; 0 builds each on
+        the scheduler thread when it reaches the head of the queue.
     """
 
     def __init__(self, hbm_budget=200e9, max_active=16, chunk_steps=48, device="cuda",
-                 shot_factory=None, stream_factory=None, plan_flags=None, batch=True):
+                 shot_factory=None, stream_factory=None, plan_flags=None, batch=True,
+                 build_ahead=4):
         self.hbm_budget = float(hbm_budget)
         self.max_active = int(max_active)
         self.chunk_steps = int(chunk_steps)
@@ -166,6 +178,10 @@ class ShotServer:
         self._submitted = 0
         self.batch = bool(batch)
         self._runners = {}            # variant -> _BatchRunner
+        self.build_ahead = max(0, int(build_ahead))
+        self._builders = (ThreadPoolExecutor(max_workers=self.build_ahead,
+                                             thread_name_prefix="shot-build")
+                          if self.build_ahead else None)
         self.stats = ServeStats()
         self._sched = threading.Thread(target=self._schedule, name="shot-scheduler",
                                        daemon=True)
@@ -196,6 +212,8 @@ class ShotServer:
             self._sched.join()
             for t in list(self._workers):
                 t.join()
+        if self._builders is not None:
+            self._builders.shutdown(wait=wait)
         with self._cv:
             runners = list(self._runners.values())
         for r in runners:
@@ -216,10 +234,22 @@ class ShotServer:
         from . import hipmem
         return hipmem.Stream(hipmem.default_library())
 
+    def _start_builds(self):
+        """Starts building the first `build_ahead` waiting requests that are
+        callables (called with the lock held; submitting does not block)."""
+        if self._builders is None:
+            return
+        for i, req in enumerate(self._waiting):
+            if i >= self.build_ahead:
+                break
+            if req.build is None and req.needs_build():
+                req.build = self._builders.submit(req.solver)
+
     def _schedule(self):
         while True:
             with self._cv:
                 while True:
+                    self._start_builds()
                     if self._waiting and self._active < self.max_active:
                         break
                     if self._closed and not self._waiting:
@@ -227,8 +257,9 @@ class ShotServer:
                     self._cv.wait()
                 req = self._waiting[0]
             try:
-                if callable(req.solver) and not hasattr(req.solver, "space_model") \
-                        and not hasattr(req.solver, "device_bytes"):
+                if req.build is not None:
+                    req.solver, req.build = req.build.result(), None
+                elif req.needs_build():
                     req.solver = req.solver()
                 need = shot_bytes(req.solver)
             except Exception as exc:          # a bad request fails alone
@@ -245,6 +276,7 @@ class ShotServer:
                     continue
                 while (self._used + need > self.hbm_budget and self._active > 0) or \
                         self._active >= self.max_active:
+                    self._start_builds()
                     self._cv.wait()
                 self._waiting.popleft()
                 self.admissions.append(req.seq)

@@ -246,10 +246,14 @@ def serving_requests(count, seed=0, n2=(256, 2048), n3=(128, 384), steps=None,
         dim, n, h = req["dim"], req["n"], 10.0
         shape = (n,) * dim
         nl = len(req["layers"])
-        vel = np.repeat(req["layers"], -(-n // nl))[:n].astype(np.float32)
-        vel = np.broadcast_to(vel.reshape((n,) + (1,) * (dim - 1)), shape).copy()
-        den = (310.0 * vel.astype(np.float64) ** 0.25).astype(np.float32) \
-            if req["dens"] else None
+        # flat layers: one value per depth row; the density of each row is
+        # formed once (the same elementwise operations as over the whole
+        # grid, so the same values) and both are copied out to full arrays
+        col = (n,) + (1,) * (dim - 1)
+        vz = np.repeat(req["layers"], -(-n // nl))[:n].astype(np.float32)
+        vel = np.broadcast_to(vz.reshape(col), shape).copy()
+        den = np.broadcast_to((310.0 * vz.astype(np.float64) ** 0.25).astype(np.float32)
+                              .reshape(col), shape).copy() if req["dens"] else None
         ext = (n - 1) * h
         sm = sw.SpaceModel(bounding_box=(0, ext) * dim, grid_spacing=(h,) * dim,
                            velocity_model=vel, density_model=den,

@@ -175,3 +175,41 @@ def test_results_dropped_on_request():
         assert got == ((("u", r.ident), ("rec", r.ident)) if r.ident % 2 == 0 else None)
     s = srv.stats.summary()
     assert s["requests"] == 6 and s["updates"] == sum(1000 * r.steps for r in reqs)
+
+
+@pytest.mark.parametrize("ahead", [0, 3])
+def test_callable_requests_are_built_once_in_order(ahead):
+    """Requests given as callables are built once each (ahead of admission
+    on the builder threads with build_ahead > 0, on the scheduler thread
+    with 0), admitted in submission order, and a build that raises fails
+    only its own request."""
+    mon = Monitor()
+    calls, threads, lock = [], set(), threading.Lock()
+
+    def builder(i):
+        def build():
+            with lock:
+                calls.append(i)
+                threads.add(threading.current_thread().name)
+            time.sleep(0.002)
+            if i == 5:
+                raise ValueError("bad model")
+            return FakeRequest(i, 10)
+        return build
+
+    with make_server(mon, hbm_budget=25, max_active=2, build_ahead=ahead) as srv:
+        futs = [srv.submit(builder(i)) for i in range(12)]
+    for i, f in enumerate(futs):
+        if i == 5:
+            with pytest.raises(ValueError, match="bad model"):
+                f.result(timeout=30)
+        else:
+            assert f.result(timeout=30)[0] == ("u", i)
+    assert sorted(calls) == list(range(12))               # each built once
+    assert mon.order == [i for i in range(12) if i != 5]  # arrival order kept
+    assert srv.admissions == [i for i in range(12) if i != 5]
+    if ahead:
+        assert all(t.startswith("shot-build") for t in threads), threads
+    else:
+        assert threads == {"shot-scheduler"}
+    assert mon.used == 0 and mon.active == 0
