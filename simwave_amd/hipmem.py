@@ -158,8 +158,8 @@ class DeviceArray:
 
     @classmethod
     def from_host(cls, lib, array, stream):
-        """A device copy of the host array `array`, enqueued on `stream`
-        (the host array must stay alive until the stream has run the copy)."""
+        """A device copy of the host array `array`, made on `stream` (which
+        is synchronised, so the host array may go at once)."""
         a = np.ascontiguousarray(array)
         out = cls(lib, a.shape, a.dtype, stream)
         out.copy_from_host(a, stream)
@@ -184,11 +184,14 @@ class DeviceArray:
         return self
 
     def copy_from_host(self, array, stream):
+        """Copies the host array in on `stream` and waits for the copy (the
+        source, perhaps a temporary, is read by the copy engine)."""
         a = np.ascontiguousarray(array, dtype=self.dtype)
         if a.nbytes != self.nbytes:
             raise ValueError("host array has %d bytes, device array %d" % (a.nbytes, self.nbytes))
         _call(self._lib, "simwave_hip_copy", _VP(self._ptr), _VP(a.ctypes.data),
               _ULL(self.nbytes), _VP(stream_handle(stream)))
+        _call(self._lib, "simwave_hip_stream_synchronize", _VP(stream_handle(stream)))
 
     def numpy(self, stream=None):
         """A host copy (synchronises `stream`)."""
